@@ -1,0 +1,13 @@
+"""pytest configuration: the ``gpu`` marker and import paths."""
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line(
+        "markers", "gpu: needs an MI355X (gfx950) GPU; parity tests through the C-ABI")
+    config.addinivalue_line("markers", "slow: takes more than a few seconds on CPU")
