@@ -1,0 +1,848 @@
+// bh_engine.hip -- handle, device memory and the C-ABI of the MI355X IMC
+// engine (include/branson_hip.h).  Kernels live in bh_kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/branson_hip.h"
+#include "bh_kernels.h"
+#include "bh_tracking.h"
+
+using namespace bh;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+inline uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Field sizes of PhotonSoA, in declaration order.
+constexpr size_t kFieldBytes[15] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 4, 4, 1, 1};
+
+// One slab of device memory holding every field of `cap` photons.
+struct PhotonStore {
+  void* slab = nullptr;
+  uint64_t cap = 0;
+  PhotonSoA soa{};
+
+  static size_t bytes_for(uint64_t cap) {
+    size_t b = 0;
+    for (size_t fb : kFieldBytes) b += align_up(fb * cap);
+    return b;
+  }
+  static PhotonSoA carve(void* slab, uint64_t cap) {
+    char* p = static_cast<char*>(slab);
+    void* f[15];
+    for (int k = 0; k < 15; ++k) {
+      f[k] = p;
+      p += align_up(kFieldBytes[k] * cap);
+    }
+    PhotonSoA s;
+    s.x = (double*)f[0];
+    s.y = (double*)f[1];
+    s.z = (double*)f[2];
+    s.u = (double*)f[3];
+    s.v = (double*)f[4];
+    s.w = (double*)f[5];
+    s.E = (double*)f[6];
+    s.E0 = (double*)f[7];
+    s.life = (double*)f[8];
+    s.ctr = (uint64_t*)f[9];
+    s.stream = (uint64_t*)f[10];
+    s.cell = (uint32_t*)f[11];
+    s.group = (uint32_t*)f[12];
+    s.desc = (uint8_t*)f[13];
+    s.stype = (uint8_t*)f[14];
+    return s;
+  }
+  static void* field(const PhotonSoA& s, int k) {
+    void* f[15] = {s.x, s.y, s.z, s.u, s.v, s.w, s.E, s.E0, s.life,
+                   s.ctr, s.stream, s.cell, s.group, s.desc, s.stype};
+    return f[k];
+  }
+};
+
+}  // namespace
+
+template <typename T>
+struct DArr {
+  T* p = nullptr;
+  size_t cap = 0;
+};
+
+struct bh_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bh_options opt{};
+  std::string err;
+
+  // mesh
+  uint32_t n_cells = 0, G = 0;
+  DArr<CellGeo> geo;
+  DArr<double2> sigma;
+  DArr<uint32_t> next;
+  bool has_mesh = false;
+
+  // per-cell source energies
+  DArr<double> em, src, cen;
+  DArr<int8_t> face;
+  bool has_energy = false, has_census_E = false;
+
+  // photons: W = [census | new], C = next census / scratch
+  PhotonStore W, C;
+  uint64_t n_cen = 0, n_new = 0;
+  uint64_t ctr_hi = 0;
+
+  // tallies
+  DArr<double> abs_E, track_E;
+
+  // scratch
+  DArr<uint64_t> counts, off, tiles, tiles2;
+  DArr<CensusPartial> parts;
+  DArr<AosPhoton> aos;
+  uint64_t* d_total = nullptr;      // scan totals
+  CensusPartial* d_ctot = nullptr;  // census totals
+  DevCounters* d_ctrs = nullptr;
+  PersistentQueue* d_queue = nullptr;
+  void* h_pinned = nullptr;  // small readbacks
+
+  hipEvent_t ev[6] = {};
+  double last_source_ms = 0.0;
+  int n_cu = 256;
+  size_t lds_budget = 48 * 1024;  // dynamic LDS a tally may use per block
+};
+
+namespace {
+
+int fail(bh_handle* h, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                             \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail((h), BH_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,               \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                       \
+  } while (0)
+
+#define TRY(expr)            \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != BH_OK) return r_; \
+  } while (0)
+
+// Makes a hold at least n elements (contents are not kept).
+template <typename T>
+int ensure(bh_handle* h, DArr<T>& a, size_t n) {
+  if (n <= a.cap && a.p) return BH_OK;
+  if (a.p) HIPCHK(h, hipFree(a.p));
+  a.p = nullptr;
+  a.cap = 0;
+  const size_t want = std::max<size_t>(n, 1);
+  if (hipMalloc(&a.p, want * sizeof(T)) != hipSuccess) {
+    a.p = nullptr;
+    return fail(h, BH_ERR_ALLOC, "device allocation of %zu bytes failed", want * sizeof(T));
+  }
+  a.cap = want;
+  return BH_OK;
+}
+
+template <typename T>
+void release(DArr<T>& a) {
+  if (a.p) (void)hipFree(a.p);
+  a.p = nullptr;
+  a.cap = 0;
+}
+
+// Grows a photon store to hold `need`, keeping the first `keep` photons.
+int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
+  if (need <= s.cap && s.slab) return BH_OK;
+  uint64_t cap = std::max<uint64_t>(need + need / 8, 1024);
+  void* slab = nullptr;
+  if (hipMalloc(&slab, PhotonStore::bytes_for(cap)) != hipSuccess)
+    return fail(h, BH_ERR_ALLOC, "device allocation for %llu photons failed",
+                (unsigned long long)cap);
+  PhotonSoA ns = PhotonStore::carve(slab, cap);
+  if (keep && s.slab) {
+    for (int k = 0; k < 15; ++k)
+      HIPCHK(h, hipMemcpyAsync(PhotonStore::field(ns, k), PhotonStore::field(s.soa, k),
+                               keep * kFieldBytes[k], hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  if (s.slab) HIPCHK(h, hipFree(s.slab));
+  s.slab = slab;
+  s.cap = cap;
+  s.soa = ns;
+  return BH_OK;
+}
+
+DevCells dev_cells(const bh_handle* h) {
+  return DevCells{h->geo.p, h->sigma.p, h->next.p, h->n_cells, h->G};
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)div_up(n, block); }
+
+// Exclusive scan of n uint64 values on the device; the total lands in
+// h->d_total[slot].
+int scan_u64(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out, int slot) {
+  const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kScanTile), 1);
+  // the single-block middle pass loops over the tile sums; keep it short
+  if (n_tiles > 65536)
+    return fail(h, BH_ERR_ARG, "scan of %llu elements exceeds the supported size",
+                (unsigned long long)n);
+  TRY(ensure(h, h->tiles, n_tiles));
+  k_scan_reduce<<<(unsigned)n_tiles, kScanNT, 0, h->stream>>>(in, n, h->tiles.p);
+  k_scan_top<<<1, kScanNT, 0, h->stream>>>(h->tiles.p, n_tiles, h->d_total + slot);
+  k_scan_down<<<(unsigned)n_tiles, kScanNT, 0, h->stream>>>(in, n, h->tiles.p, out);
+  HIPCHK(h, hipGetLastError());
+  return BH_OK;
+}
+
+int read_total(bh_handle* h, int slot, uint64_t* v) {
+  uint64_t* hp = static_cast<uint64_t*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp, h->d_total + slot, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  *v = *hp;
+  return BH_OK;
+}
+
+int check_cells(bh_handle* h, const bh_cells* c) {
+  if (!c || !c->nodes || !c->bc || !c->next || !c->sigma_a || !c->sigma_s || !c->fleck)
+    return fail(h, BH_ERR_ARG, "bh_cells has a NULL array");
+  if (c->n_cells == 0 || c->n_groups == 0)
+    return fail(h, BH_ERR_ARG, "bh_cells needs n_cells > 0 and n_groups > 0");
+  for (size_t i = 0; i < (size_t)c->n_cells * 6; ++i) {
+    if (c->bc[i] < BH_REFLECT || c->bc[i] > BH_PROCESSOR)
+      return fail(h, BH_ERR_ARG, "cell %zu face %zu: bc %d out of range", i / 6, i % 6,
+                  c->bc[i]);
+    if (c->bc[i] == BH_ELEMENT && c->next[i] >= c->n_cells)
+      return fail(h, BH_ERR_ARG, "cell %zu face %zu: neighbour %u out of range", i / 6, i % 6,
+                  c->next[i]);
+  }
+  return BH_OK;
+}
+
+// Runs `launch` with the tally sink chosen for this mesh: block-private LDS
+// tallies when both arrays fit, else global atomics.
+template <class Launch>
+int with_tally(bh_handle* h, bool lds, Launch&& launch) {
+  if (lds) {
+    LdsTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr};
+    return launch(t, (size_t)2 * h->n_cells * sizeof(double));
+  }
+  GlobalTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr};
+  return launch(t, (size_t)0);
+}
+
+bool use_lds_tally(const bh_handle* h, bool persistent) {
+  const size_t bytes = (size_t)2 * h->n_cells * sizeof(double);
+  if (h->opt.tally_mode == 1) return false;
+  if (h->opt.tally_mode == 2) return bytes <= h->lds_budget;
+  return bytes <= (persistent ? h->lds_budget : std::min<size_t>(h->lds_budget, 16 * 1024));
+}
+
+// Transport of W[0, n) with the configured kernel; counters in h->d_ctrs.
+int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n) {
+  if (n == 0) return BH_OK;
+  const DevCells cells = dev_cells(h);
+  const int block = 256;
+  const bool persistent = h->opt.algorithm == BH_ALGO_PERSISTENT;
+  const bool lds = use_lds_tally(h, persistent);
+  const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
+  if (!persistent) {
+    return with_tally(h, lds, [&](auto t, size_t shm) -> int {
+      using T = decltype(t);
+      if (fp32)
+        k_transport_history<T, float><<<grid_for(n, block), block, shm, h->stream>>>(
+            cells, W, n, h->ctr_hi, t, h->d_ctrs);
+      else
+        k_transport_history<T, double><<<grid_for(n, block), block, shm, h->stream>>>(
+            cells, W, n, h->ctr_hi, t, h->d_ctrs);
+      HIPCHK(h, hipGetLastError());
+      return BH_OK;
+    });
+  }
+  const int bpc = h->opt.blocks_per_cu > 0 ? h->opt.blocks_per_cu : 8;
+  const int refill = h->opt.refill_threshold > 0 ? std::min(h->opt.refill_threshold, kWave) : 16;
+  const unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)h->n_cu * bpc,
+                                                     std::max<uint64_t>(div_up(n, block), 1));
+  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_queue),
+                                      sizeof(PersistentQueue) / 8);
+  return with_tally(h, lds, [&](auto t, size_t shm) -> int {
+    using T = decltype(t);
+    if (fp32)
+      k_transport_persistent<T, float><<<grid, block, shm, h->stream>>>(
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill);
+    else
+      k_transport_persistent<T, double><<<grid, block, shm, h->stream>>>(
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill);
+    HIPCHK(h, hipGetLastError());
+    return BH_OK;
+  });
+}
+
+int zero_tallies_and_counters(bh_handle* h) {
+  k_fill_f64<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(h->abs_E.p, h->n_cells, 0.0);
+  k_fill_f64<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(h->track_E.p, h->n_cells, 0.0);
+  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_ctrs),
+                                      sizeof(DevCounters) / 8);
+  HIPCHK(h, hipGetLastError());
+  return BH_OK;
+}
+
+float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int bh_abi_version(void) { return BH_ABI_VERSION; }
+
+int bh_create(int device, const bh_options* opts, bh_handle** out) {
+  if (!out) return BH_ERR_ARG;
+  *out = nullptr;
+  bh_handle* h = new (std::nothrow) bh_handle();
+  if (!h) return BH_ERR_ALLOC;
+  if (opts) h->opt = *opts;
+  if (h->opt.block_size == 0) h->opt.block_size = 256;
+  if (h->opt.block_size != 256) {
+    int r = fail(h, BH_ERR_ARG, "block_size %d not supported (256)", h->opt.block_size);
+    *out = h;
+    return r;
+  }
+  if (h->opt.precision < 0 || h->opt.precision > 1 || h->opt.algorithm < 0 ||
+      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 2) {
+    int r = fail(h, BH_ERR_ARG, "invalid bh_options");
+    *out = h;
+    return r;
+  }
+  h->device = device;
+  *out = h;
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0)
+    return fail(h, BH_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n_dev)
+    return fail(h, BH_ERR_ARG, "device %d out of range (%d devices)", device, n_dev);
+  HIPCHK(h, hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(h, hipGetDeviceProperties(&prop, device));
+  h->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  // leave room for the kernels' static LDS (block scans, counters)
+  if (prop.sharedMemPerBlock > 4096)
+    h->lds_budget = std::min<size_t>(48 * 1024, prop.sharedMemPerBlock - 4096);
+  HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (auto& e : h->ev) HIPCHK(h, hipEventCreate(&e));
+  HIPCHK(h, hipMalloc(&h->d_total, 4 * sizeof(uint64_t)));
+  HIPCHK(h, hipMalloc(&h->d_ctot, sizeof(CensusPartial)));
+  HIPCHK(h, hipMalloc(&h->d_ctrs, sizeof(DevCounters)));
+  HIPCHK(h, hipMalloc(&h->d_queue, sizeof(PersistentQueue)));
+  HIPCHK(h, hipHostMalloc(&h->h_pinned, 4096, hipHostMallocDefault));
+  if (h->opt.reserve_photons) {
+    TRY(grow_store(h, h->W, h->opt.reserve_photons, 0));
+    TRY(grow_store(h, h->C, h->opt.reserve_photons, 0));
+  }
+  return BH_OK;
+}
+
+void bh_destroy(bh_handle* h) {
+  if (!h) return;
+  if (h->stream) {
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+  }
+  release(h->geo);
+  release(h->sigma);
+  release(h->next);
+  release(h->em);
+  release(h->src);
+  release(h->cen);
+  release(h->face);
+  release(h->abs_E);
+  release(h->track_E);
+  release(h->counts);
+  release(h->off);
+  release(h->tiles);
+  release(h->tiles2);
+  release(h->parts);
+  release(h->aos);
+  void* ptrs[] = {h->W.slab, h->C.slab, h->d_total, h->d_ctot, h->d_ctrs, h->d_queue};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char* bh_last_error(const bh_handle* h) {
+  if (!h) return "null handle";
+  return h->err.c_str();
+}
+
+int bh_sync(bh_handle* h) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return BH_OK;
+}
+
+int bh_set_mesh(bh_handle* h, const bh_cells* c) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  TRY(check_cells(h, c));
+  const uint32_t n = c->n_cells, G = c->n_groups;
+  std::vector<CellGeo> geo(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    CellGeo& g = geo[i];
+    std::memcpy(g.nodes, c->nodes + (size_t)i * 6, 6 * sizeof(double));
+    g.fleck = c->fleck[i];
+    uint32_t bits = 0;
+    for (int f = 0; f < 6; ++f) bits |= (uint32_t)c->bc[(size_t)i * 6 + f] << (3 * f);
+    g.bc_bits = bits;
+    g.source_face_plus1 = 0;
+    for (int f = 0; f < 6; ++f)
+      if (c->bc[(size_t)i * 6 + f] == BH_SOURCE) {
+        g.source_face_plus1 = (uint32_t)f + 1;
+        break;
+      }
+  }
+  std::vector<double2> sig((size_t)n * G);
+  for (size_t k = 0; k < (size_t)n * G; ++k) sig[k] = make_double2(c->sigma_a[k], c->sigma_s[k]);
+  HIPCHK(h, hipSetDevice(h->device));
+  TRY(ensure(h, h->geo, n));
+  TRY(ensure(h, h->next, (size_t)n * 6));
+  TRY(ensure(h, h->sigma, (size_t)n * G));
+  TRY(ensure(h, h->abs_E, n));
+  TRY(ensure(h, h->track_E, n));
+  HIPCHK(h, hipMemcpyAsync(h->geo.p, geo.data(), n * sizeof(CellGeo), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->next.p, c->next, (size_t)n * 6 * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->sigma.p, sig.data(), sig.size() * sizeof(double2),
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->has_mesh && (n != h->n_cells || G != h->G)) {
+    h->has_energy = false;  // energies were sized for the old mesh
+  }
+  h->n_cells = n;
+  h->G = G;
+  h->has_mesh = true;
+  return BH_OK;
+}
+
+int bh_set_source_energy(bh_handle* h, const double* emission_E, const double* source_E,
+                         const double* census_E, const int8_t* source_face) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_source_energy before bh_set_mesh");
+  if (!emission_E || !source_E || !source_face)
+    return fail(h, BH_ERR_ARG, "emission_E, source_E and source_face are required");
+  const uint32_t n = h->n_cells;
+  for (uint32_t c = 0; c < n; ++c)
+    if (source_E[c] > 0.0 && (source_face[c] < 0 || source_face[c] > 5))
+      return fail(h, BH_ERR_ARG, "cell %u has source energy but no source face", c);
+  HIPCHK(h, hipSetDevice(h->device));
+  TRY(ensure(h, h->em, n));
+  TRY(ensure(h, h->src, n));
+  TRY(ensure(h, h->cen, n));
+  TRY(ensure(h, h->face, n));
+  HIPCHK(h, hipMemcpyAsync(h->em.p, emission_E, n * sizeof(double), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->src.p, source_E, n * sizeof(double), hipMemcpyHostToDevice,
+                           h->stream));
+  if (census_E)
+    HIPCHK(h, hipMemcpyAsync(h->cen.p, census_E, n * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->face.p, source_face, n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->has_energy = true;
+  h->has_census_E = census_E != nullptr;
+  return BH_OK;
+}
+
+int bh_source(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_t seed,
+              uint32_t cycle, uint32_t rank, double dt, uint64_t* n_new_out) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh || !h->has_energy)
+    return fail(h, BH_ERR_STATE, "bh_source needs bh_set_mesh and bh_set_source_energy");
+  if (!(total_E > 0.0)) return fail(h, BH_ERR_ARG, "total_E must be > 0");
+  HIPCHK(h, hipSetDevice(h->device));
+  const uint32_t n = h->n_cells;
+  const DevCells cells = dev_cells(h);
+  TRY(ensure(h, h->counts, n));
+  TRY(ensure(h, h->off, n));
+  const uint64_t ctr_hi = (uint64_t)seed << 32;
+  if (h->n_cen && ctr_hi != h->ctr_hi)
+    return fail(h, BH_ERR_ARG, "seed differs from the resident census photons' seed");
+  h->ctr_hi = ctr_hi;
+  const uint64_t rank_off = n_user_photons * (uint64_t)rank;
+  HIPCHK(h, hipEventRecord(h->ev[4], h->stream));
+
+  if (h->has_census_E) {  // step 1: initial census replaces the resident one
+    k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->cen.p, nullptr, n,
+                                                              n_user_photons, total_E, 1,
+                                                              h->counts.p);
+    TRY(scan_u64(h, h->counts.p, n, h->off.p, 0));
+    uint64_t n_c = 0;
+    TRY(read_total(h, 0, &n_c));
+    TRY(grow_store(h, h->W, n_c, 0));
+    if (n_c)
+      k_source_fill<<<grid_for(n_c, 256), 256, 0, h->stream>>>(
+          cells, h->cen.p, nullptr, h->face.p, h->off.p, n_c, n_user_photons, total_E, dt,
+          rank_off, ctr_hi, 1, h->W.soa, 0);
+    HIPCHK(h, hipGetLastError());
+    h->n_cen = n_c;
+    h->has_census_E = false;
+  }
+  k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, n,
+                                                            n_user_photons, total_E, 0,
+                                                            h->counts.p);
+  TRY(scan_u64(h, h->counts.p, n, h->off.p, 1));
+  uint64_t n_new = 0;
+  TRY(read_total(h, 1, &n_new));
+  TRY(grow_store(h, h->W, h->n_cen + n_new, h->n_cen));
+  const uint64_t stream_base = 10000000000000ull * (uint64_t)cycle + rank_off;
+  if (n_new)
+    k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
+        cells, h->em.p, h->src.p, h->face.p, h->off.p, n_new, n_user_photons, total_E, dt,
+        stream_base, ctr_hi, 0, h->W.soa, h->n_cen);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->ev[5], h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->last_source_ms = elapsed_ms(h->ev[4], h->ev[5]);
+  h->n_new = n_new;
+  if (n_new_out) *n_new_out = n_new;
+  return BH_OK;
+}
+
+int bh_source_from_energy(bh_handle* h, const double* emission_E, const double* source_E,
+                          const double* census_E, const int8_t* source_face,
+                          uint64_t n_user_photons, double total_E, uint32_t seed,
+                          uint32_t cycle, uint32_t rank, double dt, uint64_t* n_new) {
+  TRY(bh_set_source_energy(h, emission_E, source_E, census_E, source_face));
+  return bh_source(h, n_user_photons, total_E, seed, cycle, rank, dt, n_new);
+}
+
+int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
+                 bh_step_stats* stats) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_transport before bh_set_mesh");
+  HIPCHK(h, hipSetDevice(h->device));
+  const uint64_t n = h->n_cen + h->n_new;
+  TRY(zero_tallies_and_counters(h));
+  HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+  TRY(launch_transport(h, h->W.soa, n));
+  HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+
+  // census compaction into C, in the reference's photon order
+  const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kCensusTile), 1);
+  TRY(ensure(h, h->tiles2, n_tiles));
+  TRY(ensure(h, h->parts, n_tiles));
+  TRY(grow_store(h, h->C, n, 0));
+  k_census_count<<<(unsigned)n_tiles, kCensusNT, 0, h->stream>>>(
+      h->W.soa, n, h->n_new, h->n_cen, h->tiles2.p, h->parts.p);
+  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, n_tiles, h->d_ctot);
+  // exclusive scan of per-tile census counts -> tile offsets
+  TRY(ensure(h, h->off, n_tiles));
+  TRY(scan_u64(h, h->tiles2.p, n_tiles, h->off.p, 2));
+  k_census_scatter<<<(unsigned)n_tiles, kCensusNT, 0, h->stream>>>(
+      h->W.soa, n, h->n_new, h->n_cen, h->off.p, h->C.soa, kC * next_dt);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+
+  // small readback: census totals + counters
+  char* hp = static_cast<char*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(hp + 256, h->d_ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                           h->stream));
+  if (abs_E)
+    HIPCHK(h, hipMemcpyAsync(abs_E, h->abs_E.p, h->n_cells * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+  if (track_E)
+    HIPCHK(h, hipMemcpyAsync(track_E, h->track_E.p, h->n_cells * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  CensusPartial tot;
+  DevCounters ctrs;
+  std::memcpy(&tot, hp, sizeof tot);
+  std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+
+  std::swap(h->W, h->C);
+  h->n_cen = tot.n_census;
+  h->n_new = 0;
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->census_E = tot.census_E;
+    stats->exit_E = tot.exit_E;
+    stats->n_transported = n;
+    stats->n_census = tot.n_census;
+    stats->n_exit = tot.n_exit;
+    stats->n_killed = tot.n_killed;
+    stats->n_segments = ctrs.segments;
+    stats->n_flushes = ctrs.flushes;
+    stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
+    stats->census_ms = elapsed_ms(h->ev[1], h->ev[2]);
+    stats->source_ms = h->last_source_ms;
+  }
+  if (ctrs.stuck)
+    return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
+                ctrs.stuck, kMaxSegments);
+  if (tot.n_other)
+    return fail(h, BH_ERR_STATE, "%llu photon(s) ended without EXIT/CENSUS/KILLED",
+                tot.n_other);
+  return BH_OK;
+}
+
+int bh_get_tallies(bh_handle* h, double* abs_E, double* track_E) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "no mesh");
+  if (abs_E)
+    HIPCHK(h, hipMemcpyAsync(abs_E, h->abs_E.p, h->n_cells * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+  if (track_E)
+    HIPCHK(h, hipMemcpyAsync(track_E, h->track_E.p, h->n_cells * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return BH_OK;
+}
+
+int bh_census_size(bh_handle* h, uint64_t* n) {
+  if (!h || !n) return BH_ERR_ARG;
+  *n = h->n_cen;
+  return BH_OK;
+}
+
+int bh_census_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
+  if (!h || !h->stream || !n) return BH_ERR_ARG;
+  *n = h->n_cen;
+  if (h->n_cen == 0) return BH_OK;
+  if (!out || cap < h->n_cen)
+    return fail(h, BH_ERR_CAPACITY, "census has %llu photons, buffer holds %llu",
+                (unsigned long long)h->n_cen, (unsigned long long)cap);
+  HIPCHK(h, hipSetDevice(h->device));
+  TRY(ensure(h, h->aos, h->n_cen));
+  k_soa_to_aos<<<grid_for(h->n_cen, 256), 256, 0, h->stream>>>(h->W.soa, 0, h->n_cen,
+                                                                h->ctr_hi, h->aos.p);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, h->aos.p, h->n_cen * sizeof(AosPhoton), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return BH_OK;
+}
+
+int bh_photons_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
+  if (!h || !h->stream || !n) return BH_ERR_ARG;
+  const uint64_t total = h->n_cen + h->n_new;
+  *n = total;
+  if (total == 0) return BH_OK;
+  if (!out || cap < total)
+    return fail(h, BH_ERR_CAPACITY, "%llu resident photons, buffer holds %llu",
+                (unsigned long long)total, (unsigned long long)cap);
+  HIPCHK(h, hipSetDevice(h->device));
+  TRY(ensure(h, h->aos, total));
+  // new photons live at W[n_cen, n_cen + n_new), census at W[0, n_cen)
+  if (h->n_new)
+    k_soa_to_aos<<<grid_for(h->n_new, 256), 256, 0, h->stream>>>(h->W.soa, h->n_cen, h->n_new,
+                                                                  h->ctr_hi, h->aos.p);
+  if (h->n_cen)
+    k_soa_to_aos<<<grid_for(h->n_cen, 256), 256, 0, h->stream>>>(h->W.soa, 0, h->n_cen,
+                                                                  h->ctr_hi, h->aos.p + h->n_new);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, h->aos.p, total * sizeof(AosPhoton), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return BH_OK;
+}
+
+static int upload_aos(bh_handle* h, const bh_photon* in, uint64_t n, PhotonStore& dst,
+                      uint64_t keep) {
+  if (n == 0) return BH_OK;
+  const uint64_t hi = in[0].rng[1];
+  for (uint64_t i = 0; i < n; ++i) {
+    if (in[i].rng[1] != hi || in[i].rng[3] != 0)
+      return fail(h, BH_ERR_ARG,
+                  "photon %llu: RNG counter-high/key-high words differ from photon 0 "
+                  "(one seed per run is required)",
+                  (unsigned long long)i);
+    if (h->has_mesh && in[i].cell >= h->n_cells)
+      return fail(h, BH_ERR_ARG, "photon %llu: cell %u out of range",
+                  (unsigned long long)i, in[i].cell);
+    if (h->has_mesh && in[i].group >= h->G)
+      return fail(h, BH_ERR_ARG, "photon %llu: group %u out of range",
+                  (unsigned long long)i, in[i].group);
+  }
+  h->ctr_hi = hi;
+  TRY(ensure(h, h->aos, n));
+  TRY(grow_store(h, dst, n, keep));
+  HIPCHK(h, hipMemcpyAsync(h->aos.p, in, n * sizeof(AosPhoton), hipMemcpyHostToDevice,
+                           h->stream));
+  k_aos_to_soa<<<grid_for(n, 256), 256, 0, h->stream>>>(h->aos.p, n, dst.soa, 0);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return BH_OK;
+}
+
+int bh_census_import(bh_handle* h, const bh_photon* in, uint64_t n) {
+  if (!h || !h->stream || (n && !in)) return BH_ERR_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  TRY(upload_aos(h, in, n, h->W, 0));
+  h->n_cen = n;
+  h->n_new = 0;
+  return BH_OK;
+}
+
+int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* abs_E,
+                         double* track_E, bh_step_stats* stats) {
+  if (!h || !h->stream || (n && !photons)) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_transport_photons before bh_set_mesh");
+  HIPCHK(h, hipSetDevice(h->device));
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (n == 0) return BH_OK;
+  const uint64_t saved_hi = h->ctr_hi;
+  TRY(upload_aos(h, photons, n, h->C, 0));
+  TRY(zero_tallies_and_counters(h));
+  HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+  TRY(launch_transport(h, h->C.soa, n));
+  HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+  k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(h->C.soa, 0, n, h->ctr_hi, h->aos.p);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(photons, h->aos.p, n * sizeof(AosPhoton), hipMemcpyDeviceToHost,
+                           h->stream));
+  std::vector<double> a(h->n_cells), t(h->n_cells);
+  HIPCHK(h, hipMemcpyAsync(a.data(), h->abs_E.p, h->n_cells * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(t.data(), h->track_E.p, h->n_cells * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+  char* hp = static_cast<char*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp + 256, h->d_ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ctr_hi = h->n_cen ? saved_hi : h->ctr_hi;
+  DevCounters ctrs;
+  std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  for (uint32_t c = 0; c < h->n_cells; ++c) {
+    if (abs_E) abs_E[c] += a[c];
+    if (track_E) track_E[c] += t[c];
+  }
+  if (stats) {
+    stats->n_transported = n;
+    stats->n_segments = ctrs.segments;
+    stats->n_flushes = ctrs.flushes;
+    stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint8_t d = photons[i].descriptor[0];
+      if (d == BH_CENSUS) {
+        stats->n_census++;
+        stats->census_E += photons[i].E;
+      } else if (d == BH_EXIT) {
+        stats->n_exit++;
+        stats->exit_E += photons[i].E;
+      } else if (d == BH_KILLED) {
+        stats->n_killed++;
+      }
+    }
+  }
+  if (ctrs.stuck)
+    return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
+                ctrs.stuck, kMaxSegments);
+  return BH_OK;
+}
+
+int bh_rng_draws(bh_handle* h, uint32_t seed, uint64_t stream, uint64_t ctr0, uint32_t n,
+                 double* out) {
+  if (!h || !h->stream || (n && !out)) return BH_ERR_ARG;
+  if (n == 0) return BH_OK;
+  if (ctr0 + n < ctr0)
+    return fail(h, BH_ERR_ARG, "counter range wraps the 64-bit low word");
+  HIPCHK(h, hipSetDevice(h->device));
+  double* d = nullptr;
+  HIPCHK(h, hipMalloc(&d, n * sizeof(double)));
+  k_rng_draws<<<grid_for(n, 256), 256, 0, h->stream>>>((uint64_t)seed << 32, stream, ctr0, n,
+                                                        d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(h, BH_ERR_DEVICE, "bh_rng_draws: %s", hipGetErrorString(e));
+  return BH_OK;
+}
+
+int bh_copy_probe(bh_handle* h, uint64_t n_bytes, int reps, double* gbps) {
+  if (!h || !h->stream || !gbps || reps <= 0 || n_bytes < 16) return BH_ERR_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  const uint64_t n = n_bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  HIPCHK(h, hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(h, BH_ERR_ALLOC, "bh_copy_probe: allocation failed");
+  }
+  const unsigned grid = (unsigned)std::min<uint64_t>(div_up(n, 256), (uint64_t)h->n_cu * 16);
+  hipError_t e = hipMemsetAsync(a, 0, n * 16, h->stream);
+  if (e == hipSuccess) {
+    k_copy_probe<<<grid, 256, 0, h->stream>>>(a, b, n);  // warm-up
+    e = hipEventRecord(h->ev[4], h->stream);
+  }
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    k_copy_probe<<<grid, 256, 0, h->stream>>>(a, b, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(h->ev[5], h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  const double ms = elapsed_ms(h->ev[4], h->ev[5]) / reps;
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(h, BH_ERR_DEVICE, "bh_copy_probe: %s", hipGetErrorString(e));
+  *gbps = ms > 0 ? 2.0 * (double)(n * 16) / (ms * 1e-3) / 1e9 : 0.0;
+  return BH_OK;
+}
+
+int bh_threefry(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out) {
+  if (!h || !h->stream || (n && (!in || !out))) return BH_ERR_ARG;
+  if (n == 0) return BH_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  uint64_t *di = nullptr, *dout = nullptr;
+  HIPCHK(h, hipMalloc(&di, n * 4 * sizeof(uint64_t)));
+  if (hipMalloc(&dout, n * 2 * sizeof(uint64_t)) != hipSuccess) {
+    (void)hipFree(di);
+    return fail(h, BH_ERR_ALLOC, "bh_threefry: allocation failed");
+  }
+  hipError_t e = hipMemcpyAsync(di, in, n * 4 * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                h->stream);
+  if (e == hipSuccess) {
+    k_threefry<<<grid_for(n, 256), 256, 0, h->stream>>>(di, n, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, dout, n * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(di);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(h, BH_ERR_DEVICE, "bh_threefry: %s", hipGetErrorString(e));
+  return BH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,534 @@
+// bh_kernels.h -- kernels of the MI355X IMC engine (source, tracking, census,
+// scans) and the device-side data layout they share.  Included once by
+// bh_engine.hip.
+#pragma once
+
+#include "bh_device.h"
+
+namespace bh {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Photon state in HBM: structure of arrays, one 8/4/1-byte stream per field,
+// so every per-photon load and store of a wavefront is one coalesced
+// transaction.  98 bytes per photon.
+// ---------------------------------------------------------------------------
+struct PhotonSoA {
+  double *x, *y, *z;  // position (cm)
+  double *u, *v, *w;  // direction cosines
+  double *E, *E0, *life;
+  uint64_t *ctr, *stream;
+  uint32_t *cell, *group;
+  uint8_t *desc, *stype;
+};
+
+struct DevCells {
+  const CellGeo* geo;     // [n_cells]
+  const double2* sigma;   // [n_cells * G] (sigma_a, sigma_s)
+  const uint32_t* next;   // [n_cells * 6]
+  uint32_t n_cells;
+  uint32_t G;
+};
+
+struct DevCounters {
+  unsigned long long segments;
+  unsigned long long flushes;
+  unsigned long long stuck;
+  unsigned long long pad;
+};
+
+struct CensusPartial {  // per block of the census compaction
+  double census_E;
+  double exit_E;
+  unsigned long long n_census;
+  unsigned long long n_exit;
+  unsigned long long n_killed;
+  unsigned long long n_other;
+};
+
+// ---------------------------------------------------------------------------
+// Block scan helpers (blockDim.x == 256: four wave64s).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const T o = __shfl_up(v, d, kWave);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ T block_excl_scan(T v, T* lds /*[NT/64]*/, T& total) {
+  constexpr int NW = NT / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const T incl = wave_incl_scan(v);
+  if (lane == kWave - 1) lds[wave] = incl;
+  __syncthreads();
+  T off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const T c = lds[w];
+    if (w < wave) off += c;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return off + incl - v;
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ T block_sum(T v, T* lds) {
+  T tot;
+  block_excl_scan<T, NT>(v, lds, tot);
+  return tot;
+}
+
+// Exclusive prefix over a 64-bit flag ballot: position of this lane's flag
+// among the set flags of its wave.
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+  const uint32_t lo = (uint32_t)mask, hi = (uint32_t)(mask >> 32);
+  return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+}
+
+// ---------------------------------------------------------------------------
+// Generic exclusive scan of uint64 counts: reduce per tile, scan the tile
+// sums in one block, then scan each tile with its offset.
+// ---------------------------------------------------------------------------
+constexpr int kScanNT = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanNT * kScanItems;
+
+__global__ __launch_bounds__(kScanNT) void k_scan_reduce(const uint64_t* __restrict__ in,
+                                                         uint64_t n,
+                                                         uint64_t* __restrict__ tile_sum) {
+  __shared__ uint64_t lds[kScanNT / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint64_t s = 0;
+#pragma unroll
+  for (int it = 0; it < kScanItems; ++it) {
+    const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  s = block_sum<uint64_t, kScanNT>(s, lds);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = s;
+}
+
+// One block of 256 threads scans n_tiles sums in place (exclusive) and
+// writes the grand total to *total.
+__global__ __launch_bounds__(kScanNT) void k_scan_top(uint64_t* __restrict__ tile_sum,
+                                                      uint64_t n_tiles,
+                                                      uint64_t* __restrict__ total) {
+  __shared__ uint64_t lds[kScanNT / kWave];
+  uint64_t carry = 0;
+  for (uint64_t b = 0; b < n_tiles; b += kScanNT) {
+    const uint64_t i = b + threadIdx.x;
+    const uint64_t v = (i < n_tiles) ? tile_sum[i] : 0;
+    uint64_t tot;
+    const uint64_t ex = block_excl_scan<uint64_t, kScanNT>(v, lds, tot);
+    if (i < n_tiles) tile_sum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kScanNT) void k_scan_down(const uint64_t* __restrict__ in,
+                                                       uint64_t n,
+                                                       const uint64_t* __restrict__ tile_off,
+                                                       uint64_t* __restrict__ out) {
+  __shared__ uint64_t lds[kScanNT / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint64_t carry = tile_off[blockIdx.x];
+#pragma unroll
+  for (int it = 0; it < kScanItems; ++it) {
+    const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
+    const uint64_t v = (i < n) ? in[i] : 0;
+    uint64_t tot;
+    const uint64_t ex = block_excl_scan<uint64_t, kScanNT>(v, lds, tot);
+    if (i < n) out[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, uint64_t n, double v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Source (src/source.h).
+// Photons per cell = max(1, int(n_user * E / E_tot)) when E > 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t photons_for(uint64_t n_user, double E, double total_E) {
+  if (!(E > 0.0)) return 0u;
+  uint32_t n = (uint32_t)(int)((double)n_user * E / total_E);
+  return n ? n : 1u;
+}
+
+// counts[c] = photons of cell c: emission + face source (mode 0) or initial
+// census (mode 1, emission_E then holds the census energy).
+__global__ void k_source_counts(const double* __restrict__ emission_E,
+                                const double* __restrict__ source_E, uint32_t n_cells,
+                                uint64_t n_user, double total_E, int mode,
+                                uint64_t* __restrict__ counts) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  uint64_t n = photons_for(n_user, emission_E[c], total_E);
+  if (mode == 0) n += photons_for(n_user, source_E[c], total_E);
+  counts[c] = n;
+}
+
+// One work-item per new photon p: locate its cell in the exclusive prefix
+// `off` (last cell with off[c] <= p), then sample it.  Stream number is
+// stream_base + p: 1e13*cycle + n_user*rank + p for new photons
+// (src/source.h:221-222, 267, 283), n_user*rank + p for the initial census
+// (src/source.h:144, 171).  Output goes to slot out_base + p of W.
+__global__ __launch_bounds__(256) void k_source_fill(
+    DevCells cells, const double* __restrict__ emission_E, const double* __restrict__ source_E,
+    const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off, uint64_t n_new,
+    uint64_t n_user, double total_E, double dt, uint64_t stream_base, uint64_t ctr_hi,
+    int mode, PhotonSoA W, uint64_t out_base) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_new) return;
+  uint32_t lo = 0, hi = cells.n_cells;  // upper_bound(off, p) - 1
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= p)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  const uint32_t c = lo - 1;
+  const uint64_t local = p - off[c];
+  const CellGeo& g = cells.geo[c];
+  const double nd[6] = {g.nodes[0], g.nodes[1], g.nodes[2], g.nodes[3], g.nodes[4], g.nodes[5]};
+
+  Rng rng{0ull, stream_base + p, ctr_hi};
+  double x, y, z, u, v, w, E, life;
+  uint8_t stype;
+  const uint32_t n_em = photons_for(n_user, emission_E[c], total_E);
+  if (mode == 1) {  // set_initial_census_photon (src/source.h:65-81)
+    E = emission_E[c] / (double)n_em;
+    pos_in_cell(nd, rng, x, y, z);
+    uniform_angle(rng, u, v, w);
+    life = kC * dt;
+    stype = 0;
+  } else if (local < n_em) {  // set_emission_photon (src/source.h:27-43)
+    E = emission_E[c] / (double)n_em;
+    pos_in_cell(nd, rng, x, y, z);
+    uniform_angle(rng, u, v, w);
+    life = rng.next() * kC * dt;
+    stype = 2;
+  } else {  // set_boundary_source_photon (src/source.h:46-62)
+    const uint32_t n_src = photons_for(n_user, source_E[c], total_E);
+    E = source_E[c] / (double)n_src;
+    const int face = face_of[c];
+    pos_on_face(nd, face, rng, x, y, z);
+    source_angle(face, rng, u, v, w);
+    life = rng.next() * kC * dt;
+    stype = 1;
+  }
+  const uint32_t group = (uint32_t)floor(rng.next() * (double)cells.G);
+  const uint64_t o = out_base + p;
+  W.x[o] = x;
+  W.y[o] = y;
+  W.z[o] = z;
+  W.u[o] = u;
+  W.v[o] = v;
+  W.w[o] = w;
+  W.E[o] = E;
+  W.E0[o] = E;
+  W.life[o] = life;
+  W.ctr[o] = rng.ctr;
+  W.stream[o] = rng.stream;
+  W.cell[o] = c;
+  W.group[o] = group;
+  W.desc[o] = kPass;
+  W.stype[o] = stype;
+}
+
+// ---------------------------------------------------------------------------
+// Tally sinks.  A flush adds a photon's accumulated (absorbed, track) energy
+// of one cell visit.  GlobalTally: native global_atomic_add_f64 at L2.
+// LdsTally: block-private tallies in LDS (ds_add_f64), added to HBM once per
+// block -- for meshes small enough that every block can hold them, where
+// global atomics would serialise on a few hot cells.
+// ---------------------------------------------------------------------------
+struct GlobalTally {
+  static constexpr bool kLds = false;
+  double* abs_E;
+  double* track_E;
+  double* s_abs;  // unused
+  double* s_track;
+  __device__ __forceinline__ void init(uint32_t) {}
+  __device__ __forceinline__ void add(uint32_t c, double a, double t) {
+    unsafeAtomicAdd(&abs_E[c], a);
+    unsafeAtomicAdd(&track_E[c], t);
+  }
+  __device__ __forceinline__ void finish(uint32_t) {}
+};
+
+struct LdsTally {
+  static constexpr bool kLds = true;
+  double* abs_E;  // global destination
+  double* track_E;
+  double* s_abs;  // dynamic LDS, n_cells each
+  double* s_track;
+  __device__ __forceinline__ void init(uint32_t n_cells) {
+    for (uint32_t c = threadIdx.x; c < n_cells; c += blockDim.x) {
+      s_abs[c] = 0.0;
+      s_track[c] = 0.0;
+    }
+    __syncthreads();
+  }
+  __device__ __forceinline__ void add(uint32_t c, double a, double t) {
+    __hip_atomic_fetch_add(&s_abs[c], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&s_track[c], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ __forceinline__ void finish(uint32_t n_cells) {
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < n_cells; c += blockDim.x) {
+      const double a = s_abs[c], t = s_track[c];
+      if (a != 0.0 || t != 0.0) {
+        unsafeAtomicAdd(&abs_E[c], a);
+        unsafeAtomicAdd(&track_E[c], t);
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Census compaction (post_process_photons, src/post_process_functions.h:
+// 32-59).  Logical order is the reference's photon order -- new photons
+// first, then the previous census -- while W stores [census | new]:
+// logical k maps to physical (k < n_new) ? n_cen + k : k - n_new.
+// Census photons are written in logical order to C with life = c*next_dt.
+// ---------------------------------------------------------------------------
+constexpr int kCensusNT = 256;
+constexpr int kCensusItems = 4;
+constexpr int kCensusTile = kCensusNT * kCensusItems;
+
+__device__ __forceinline__ uint64_t logical_to_phys(uint64_t k, uint64_t n_new, uint64_t n_cen) {
+  return (k < n_new) ? n_cen + k : k - n_new;
+}
+
+__global__ __launch_bounds__(kCensusNT) void k_census_count(PhotonSoA W, uint64_t n,
+                                                            uint64_t n_new, uint64_t n_cen,
+                                                            uint64_t* __restrict__ tile_count,
+                                                            CensusPartial* __restrict__ part) {
+  __shared__ double ldsd[kCensusNT / kWave];
+  __shared__ unsigned long long ldsu[kCensusNT / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kCensusTile;
+  double cE = 0.0, xE = 0.0;
+  unsigned long long nc = 0, nx = 0, nk = 0, no = 0;
+#pragma unroll
+  for (int it = 0; it < kCensusItems; ++it) {
+    const uint64_t k = base + (uint64_t)it * kCensusNT + threadIdx.x;
+    if (k < n) {
+      const uint64_t i = logical_to_phys(k, n_new, n_cen);
+      const uint8_t d = W.desc[i];
+      if (d == kCensus) {
+        ++nc;
+        cE += W.E[i];
+      } else if (d == kExit) {
+        ++nx;
+        xE += W.E[i];
+      } else if (d == kKilled) {
+        ++nk;
+      } else {
+        ++no;
+      }
+    }
+  }
+  cE = block_sum<double, kCensusNT>(cE, ldsd);
+  xE = block_sum<double, kCensusNT>(xE, ldsd);
+  nc = block_sum<unsigned long long, kCensusNT>(nc, ldsu);
+  nx = block_sum<unsigned long long, kCensusNT>(nx, ldsu);
+  nk = block_sum<unsigned long long, kCensusNT>(nk, ldsu);
+  no = block_sum<unsigned long long, kCensusNT>(no, ldsu);
+  if (threadIdx.x == 0) {
+    tile_count[blockIdx.x] = nc;
+    part[blockIdx.x] = CensusPartial{cE, xE, nc, nx, nk, no};
+  }
+}
+
+// Single block: deterministic in-order reduction of the per-tile partials.
+__global__ __launch_bounds__(256) void k_census_totals(const CensusPartial* __restrict__ part,
+                                                       uint64_t n_tiles,
+                                                       CensusPartial* __restrict__ out) {
+  __shared__ double ldsd[4];
+  __shared__ unsigned long long ldsu[4];
+  double cE = 0.0, xE = 0.0;
+  unsigned long long nc = 0, nx = 0, nk = 0, no = 0;
+  // each thread sums a contiguous run, then runs combine in thread order
+  const uint64_t per = (n_tiles + 255) / 256;
+  const uint64_t b0 = threadIdx.x * per;
+  for (uint64_t b = b0; b < b0 + per && b < n_tiles; ++b) {
+    cE += part[b].census_E;
+    xE += part[b].exit_E;
+    nc += part[b].n_census;
+    nx += part[b].n_exit;
+    nk += part[b].n_killed;
+    no += part[b].n_other;
+  }
+  cE = block_sum<double, 256>(cE, ldsd);
+  xE = block_sum<double, 256>(xE, ldsd);
+  nc = block_sum<unsigned long long, 256>(nc, ldsu);
+  nx = block_sum<unsigned long long, 256>(nx, ldsu);
+  nk = block_sum<unsigned long long, 256>(nk, ldsu);
+  no = block_sum<unsigned long long, 256>(no, ldsu);
+  if (threadIdx.x == 0) *out = CensusPartial{cE, xE, nc, nx, nk, no};
+}
+
+__global__ __launch_bounds__(kCensusNT) void k_census_scatter(
+    PhotonSoA W, uint64_t n, uint64_t n_new, uint64_t n_cen,
+    const uint64_t* __restrict__ tile_off, PhotonSoA C, double census_life) {
+  __shared__ uint32_t lds[kCensusNT / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kCensusTile;
+  uint64_t carry = tile_off[blockIdx.x];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int it = 0; it < kCensusItems; ++it) {
+    const uint64_t k = base + (uint64_t)it * kCensusNT + threadIdx.x;
+    uint64_t i = 0;
+    bool flag = false;
+    if (k < n) {
+      i = logical_to_phys(k, n_new, n_cen);
+      flag = (W.desc[i] == kCensus);
+    }
+    const uint64_t mask = __ballot(flag);
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kCensusNT / kWave; ++w) {
+      const uint32_t c = lds[w];
+      if (w < wave) woff += c;
+      tot += c;
+    }
+    __syncthreads();
+    if (flag) {
+      const uint64_t o = carry + woff + lane_rank(mask);
+      C.x[o] = W.x[i];
+      C.y[o] = W.y[i];
+      C.z[o] = W.z[i];
+      C.u[o] = W.u[i];
+      C.v[o] = W.v[i];
+      C.w[o] = W.w[i];
+      C.E[o] = W.E[i];
+      C.E0[o] = W.E0[i];
+      C.life[o] = census_life;
+      C.ctr[o] = W.ctr[i];
+      C.stream[o] = W.stream[i];
+      C.cell[o] = W.cell[i];
+      C.group[o] = W.group[i];
+      C.desc[o] = kCensus;
+      C.stype[o] = W.stype[i];
+    }
+    carry += tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// AOS <-> SoA for the reference's 120-byte Photon.
+// ---------------------------------------------------------------------------
+struct AosPhoton {
+  uint32_t cell, group, source_type;
+  uint8_t descriptor[4];
+  double pos[3];
+  double angle[3];
+  double E, E0, life_dx;
+  uint64_t rng[4];
+};
+static_assert(sizeof(AosPhoton) == 120, "reference Photon is 120 bytes");
+
+__global__ void k_aos_to_soa(const AosPhoton* __restrict__ a, uint64_t n, PhotonSoA W,
+                             uint64_t base) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const AosPhoton p = a[i];
+  const uint64_t o = base + i;
+  W.x[o] = p.pos[0];
+  W.y[o] = p.pos[1];
+  W.z[o] = p.pos[2];
+  W.u[o] = p.angle[0];
+  W.v[o] = p.angle[1];
+  W.w[o] = p.angle[2];
+  W.E[o] = p.E;
+  W.E0[o] = p.E0;
+  W.life[o] = p.life_dx;
+  W.ctr[o] = p.rng[0];
+  W.stream[o] = p.rng[2];
+  W.cell[o] = p.cell;
+  W.group[o] = p.group;
+  W.desc[o] = p.descriptor[0];
+  W.stype[o] = (uint8_t)p.source_type;
+}
+
+__global__ void k_soa_to_aos(PhotonSoA W, uint64_t base, uint64_t n, uint64_t ctr_hi,
+                             AosPhoton* __restrict__ a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t s = base + i;
+  AosPhoton p;
+  p.cell = W.cell[s];
+  p.group = W.group[s];
+  p.source_type = W.stype[s];
+  p.descriptor[0] = W.desc[s];
+  p.descriptor[1] = p.descriptor[2] = p.descriptor[3] = 0;
+  p.pos[0] = W.x[s];
+  p.pos[1] = W.y[s];
+  p.pos[2] = W.z[s];
+  p.angle[0] = W.u[s];
+  p.angle[1] = W.v[s];
+  p.angle[2] = W.w[s];
+  p.E = W.E[s];
+  p.E0 = W.E0[s];
+  p.life_dx = W.life[s];
+  p.rng[0] = W.ctr[s];
+  p.rng[1] = ctr_hi;
+  p.rng[2] = W.stream[s];
+  p.rng[3] = 0;
+  a[i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// RNG probes: device draws and raw Threefry words.
+// ---------------------------------------------------------------------------
+__global__ void k_rng_draws(uint64_t seed_hi, uint64_t stream, uint64_t ctr0, uint32_t n,
+                            double* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Rng r{ctr0 + i, stream, seed_hi};
+  out[i] = r.next();
+}
+
+// Streaming copy: 16-byte loads and stores, grid-stride.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ src,
+                                                     double2* __restrict__ dst, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+__global__ void k_threefry(const uint64_t* __restrict__ in, uint64_t n,
+                           uint64_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t o0, o1;
+  Threefry::block(in[4 * i], in[4 * i + 1], in[4 * i + 2], in[4 * i + 3], o0, o1);
+  out[2 * i] = o0;
+  out[2 * i + 1] = o1;
+}
+
+}  // namespace bh

@@ -1,0 +1,208 @@
+/*
+ * branson_hip.h -- C-ABI of the MI355X (gfx950) replicated-mode IMC
+ * transport engine.
+ *
+ * The engine replaces the transport phase of Branson's replicated driver,
+ * src/replicated_driver.h:66-88 (source sampling, join with the census,
+ * replicated_transport, post-processing into the census), with hand-written
+ * HIP kernels.  Plain pointers and sizes only; every entry point returns
+ * BH_OK (0) or a negative BH_ERR_* code and never aborts the process; the
+ * message of the last failure is available from bh_last_error().
+ *
+ * Ownership: the handle owns all device buffers; the photon census stays
+ * device-resident between steps.  The caller owns every host array it
+ * passes.  A handle is not thread safe: one handle per host thread / rank /
+ * device.  Every call is synchronous at return unless stated otherwise.
+ *
+ * Array layouts are cell-major, matching a walk over Branson's
+ * std::vector<Cell> (src/cell.h:319-336): nodes[c*6 + face],
+ * bc[c*6 + face], next[c*6 + face], sigma_a[c*n_groups + g].
+ * Face order is X_NEG, X_POS, Y_NEG, Y_POS, Z_NEG, Z_POS (src/constants.h:27).
+ */
+#ifndef BRANSON_HIP_H
+#define BRANSON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_ABI_VERSION 1
+
+/* return codes */
+#define BH_OK 0
+#define BH_ERR_ARG (-1)      /* invalid argument or size                   */
+#define BH_ERR_DEVICE (-2)   /* HIP runtime / device failure               */
+#define BH_ERR_ALLOC (-3)    /* device or host allocation failed           */
+#define BH_ERR_STATE (-4)    /* call out of order (e.g. no mesh yet)       */
+#define BH_ERR_CAPACITY (-5) /* caller buffer too small                    */
+
+/* Constants::bc_type and Constants::event_type (src/constants.h:26-28) */
+enum { BH_REFLECT = 0, BH_VACUUM = 1, BH_ELEMENT = 2, BH_SOURCE = 3, BH_PROCESSOR = 4 };
+enum { BH_EXIT = 0, BH_PASS = 1, BH_CENSUS = 2, BH_SCATTER = 3, BH_KILLED = 4, BH_BOUND = 5 };
+
+/* arithmetic of the tracking loop */
+enum {
+  BH_PRECISION_FP64 = 0,      /* everything FP64: replays the reference     */
+  BH_PRECISION_FP32_TRACK = 1 /* FP32 position/direction/path, FP64 energy  */
+};
+/* loop organisation */
+enum {
+  BH_ALGO_HISTORY = 0,   /* one work-item per photon, loop to completion  */
+  BH_ALGO_PERSISTENT = 1 /* persistent waves refill finished lanes        */
+};
+
+typedef struct bh_options {
+  int32_t precision;     /* BH_PRECISION_*                                  */
+  int32_t algorithm;     /* BH_ALGO_*                                       */
+  int32_t block_size;    /* work-items per block, 0 = default (256)         */
+  int32_t blocks_per_cu; /* persistent kernel occupancy, 0 = default        */
+  int32_t tally_mode;    /* 0 auto, 1 global atomics, 2 block-private LDS    */
+  int32_t refill_threshold; /* persistent: idle lanes of a wave that trigger
+                               a refill (1..64), 0 = default (16)          */
+  uint64_t reserve_photons; /* initial photon capacity hint, 0 = none      */
+} bh_options;
+
+/* Read-only cell data for one step (src/cell.h:319-336). */
+typedef struct bh_cells {
+  uint32_t n_cells;
+  uint32_t n_groups;
+  const double *nodes;   /* [n_cells*6]  x_lo x_hi y_lo y_hi z_lo z_hi      */
+  const int32_t *bc;     /* [n_cells*6]  BH_REFLECT .. BH_PROCESSOR         */
+  const uint32_t *next;  /* [n_cells*6]  neighbour global index             */
+  const double *sigma_a; /* [n_cells*n_groups] absorption opacity (1/cm)    */
+  const double *sigma_s; /* [n_cells*n_groups] scattering opacity (1/cm)    */
+  const double *fleck;   /* [n_cells] Fleck factor                          */
+} bh_cells;
+
+/* Diagnostics of one transport call (the IMC_State setters of
+ * src/replicated_transport.h:151-155 plus counters the reference lacks). */
+typedef struct bh_step_stats {
+  double census_E;        /* post-transport census energy                 */
+  double exit_E;          /* energy that left through VACUUM/SOURCE faces */
+  uint64_t n_transported; /* photons transported (new + census)           */
+  uint64_t n_census;      /* new census size                              */
+  uint64_t n_exit;
+  uint64_t n_killed;
+  uint64_t n_segments;    /* iterations of the tracking loop              */
+  uint64_t n_flushes;     /* tally flushes (cell exits and terminations)  */
+  double transport_ms;    /* device time of the tracking kernel           */
+  double census_ms;       /* device time of census compaction             */
+  double source_ms;       /* device time of the last bh_source call       */
+} bh_step_stats;
+
+/* A photon in the reference's 120-byte AOS layout (src/photon.h:172-181). */
+typedef struct bh_photon {
+  uint32_t cell;
+  uint32_t group;
+  uint32_t source_type;
+  uint8_t descriptor[4];
+  double pos[3];
+  double angle[3];
+  double E;
+  double E0;
+  double life_dx;
+  uint64_t rng[4]; /* counter lo, counter hi (seed<<32), key lo (stream), key hi (0) */
+} bh_photon;
+
+typedef struct bh_handle bh_handle;
+
+/* Library version / ABI check.  No counterpart in the reference. */
+int bh_abi_version(void);
+
+/* Creates a handle on HIP device `device` (the rank -> device mapping of
+ * GPU_Setup, src/gpu_setup.h:68-85, is the caller's).  Replaces the
+ * per-step GPU_Setup construction at src/replicated_driver.h:64.
+ * opts may be NULL (defaults). */
+int bh_create(int device, const bh_options *opts, bh_handle **out);
+void bh_destroy(bh_handle *h);
+const char *bh_last_error(const bh_handle *h);
+/* Waits for all work of the handle. */
+int bh_sync(bh_handle *h);
+
+/* Uploads the cells for the coming step (opacities and Fleck factor change
+ * every step, src/mesh.h:265-270).  Replaces GPU_Setup's device copy of the
+ * cell vector (src/gpu_setup.h:19-90). */
+int bh_set_mesh(bh_handle *h, const bh_cells *cells);
+
+/* Uploads the per-cell source energies of the step computed by
+ * Mesh::calculate_photon_energy (src/mesh.h:272-280): emission, face source
+ * and (first step only, else NULL) initial census energy, plus each cell's
+ * first SOURCE face or -1 (Cell::get_source_face, src/cell.h:69-76). */
+int bh_set_source_energy(bh_handle *h, const double *emission_E,
+                         const double *source_E, const double *census_E,
+                         const int8_t *source_face);
+
+/* Samples the step's photons on the device: the initial census when census
+ * energy was uploaded (make_initial_census_photons, src/source.h:134-180),
+ * then emission and face-source photons (make_photons, src/source.h:
+ * 207-291), joined ahead of the resident census (join_photon_arrays,
+ * src/census_functions.h:20-29).  cycle is the step number; total_E the
+ * all-rank photon energy (src/replicated_driver.h:57-59).
+ * *n_new (may be NULL) receives the number of new photons. */
+int bh_source(bh_handle *h, uint64_t n_user_photons, double total_E,
+              uint32_t seed, uint32_t cycle, uint32_t rank, double dt,
+              uint64_t *n_new);
+
+/* Convenience form of the two calls above with the argument list of the
+ * reference's source phase. */
+int bh_source_from_energy(bh_handle *h, const double *emission_E,
+                          const double *source_E, const double *census_E,
+                          const int8_t *source_face, uint64_t n_user_photons,
+                          double total_E, uint32_t seed, uint32_t cycle,
+                          uint32_t rank, double dt, uint64_t *n_new);
+
+/* Transports every resident photon (new + census), tallies per-cell absorbed
+ * and track-length energy, and rebuilds the census in transport order with
+ * life_dx = c * next_dt.  Replaces replicated_transport
+ * (src/replicated_transport.h:33-158).  abs_E / track_E (n_cells each) are
+ * overwritten when non-NULL; stats may be NULL. */
+int bh_transport(bh_handle *h, double next_dt, double *abs_E, double *track_E,
+                 bh_step_stats *stats);
+
+/* Copies the last step's tallies out (n_cells doubles each; either may be
+ * NULL).  Lets a caller keep the copy out of a timed transport call. */
+int bh_get_tallies(bh_handle *h, double *abs_E, double *track_E);
+
+/* Number of resident census photons. */
+int bh_census_size(bh_handle *h, uint64_t *n);
+/* Copies the census out in the reference's 120-byte layout.  *n receives the
+ * census size; fails with BH_ERR_CAPACITY when cap is smaller. */
+int bh_census_export(bh_handle *h, bh_photon *out, uint64_t cap, uint64_t *n);
+/* Copies every resident photon out in the order transport will see them --
+ * the step's new photons first, then the census (src/replicated_driver.h:
+ * 73-75) -- in the reference's 120-byte layout.  Between bh_source and
+ * bh_transport this is the joined all_photons array. */
+int bh_photons_export(bh_handle *h, bh_photon *out, uint64_t cap, uint64_t *n);
+/* Replaces the resident census (e.g. a census made by the reference). */
+int bh_census_import(bh_handle *h, const bh_photon *in, uint64_t n);
+
+/* Drop-in for gpu_transport_photons (src/history_based_transport.h:
+ * 348-413): transports the caller's AOS photons in place against the cells
+ * of the last bh_set_mesh and accumulates into abs_E/track_E (n_cells each,
+ * added to, not overwritten).  Does not touch the resident census. */
+int bh_transport_photons(bh_handle *h, bh_photon *photons, uint64_t n,
+                         double *abs_E, double *track_E, bh_step_stats *stats);
+
+/* Counter-based RNG as the device computes it: n draws of RNG(seed, stream)
+ * from counter ctr0 (src/RNG.h:262-342).  Runs on the device. */
+int bh_rng_draws(bh_handle *h, uint32_t seed, uint64_t stream, uint64_t ctr0,
+                 uint32_t n, double *out);
+/* Bandwidth probe (no counterpart in the reference): copies n_bytes between
+ * two device buffers with a plain streaming kernel (k_copy_probe), reps times,
+ * and returns the mean rate in GB/s (read + write bytes / kernel time).
+ * Used to confirm the HBM peak on a machine and to calibrate hardware byte
+ * counters against a kernel of known traffic. */
+int bh_copy_probe(bh_handle *h, uint64_t n_bytes, int reps, double *gbps);
+
+/* Raw Threefry2x64-20 words on the device for n (ctr, key) pairs
+ * (src/random123/threefry.h:190-297): in[4*i..4*i+3] = ctr0 ctr1 key0 key1,
+ * out[2*i..2*i+1]. */
+int bh_threefry(bh_handle *h, const uint64_t *in, uint64_t n, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

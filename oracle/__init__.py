@@ -178,6 +178,7 @@ def lib() -> C.CDLL:
         L.orc_sim_census_size.argtypes = [P]
         L.orc_sim_census_size.restype = U64
         L.orc_sim_get_census.argtypes = [P, P]
+        L.orc_sim_clock.argtypes = [P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -461,6 +462,13 @@ class Sim:
 
     def prepare(self):
         lib().orc_sim_prepare(self._h)
+
+    def clock(self) -> dict:
+        """time, dt, next_dt and step number of the coming step."""
+        t, dt, nd = C.c_double(), C.c_double(), C.c_double()
+        st = C.c_uint32()
+        lib().orc_sim_clock(self._h, C.byref(t), C.byref(dt), C.byref(nd), C.byref(st))
+        return {"time": t.value, "dt": dt.value, "next_dt": nd.value, "step": st.value}
 
     def step(self, n_threads: int = 1) -> dict:
         r = _StepReport()

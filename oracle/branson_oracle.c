@@ -936,6 +936,14 @@ void orc_sim_get_temperature(const orc_sim *s, double *T_e, double *T_r) {
 
 uint64_t orc_sim_census_size(const orc_sim *s) { return s->n_census; }
 
+void orc_sim_clock(const orc_sim *s, double *time, double *dt, double *next_dt,
+                   uint32_t *step) {
+  if (time) *time = s->time;
+  if (dt) *dt = s->dt;
+  if (next_dt) *next_dt = orc_next_dt(s);
+  if (step) *step = s->step;
+}
+
 void orc_sim_get_census(const orc_sim *s, orc_photon *out) {
   if (s->n_census) memcpy(out, s->census, sizeof(orc_photon) * s->n_census);
 }

@@ -183,6 +183,10 @@ void orc_sim_get_tallies(const orc_sim *sim, double *abs_E, double *track_E);
 void orc_sim_get_temperature(const orc_sim *sim, double *T_e, double *T_r);
 uint64_t orc_sim_census_size(const orc_sim *sim);
 void orc_sim_get_census(const orc_sim *sim, orc_photon *out);
+/* clock of the coming step: time, dt, the census life step next_dt
+ * (IMC_State::get_next_dt, src/imc_state.h:112-125) and the step number */
+void orc_sim_clock(const orc_sim *sim, double *time, double *dt, double *next_dt,
+                   uint32_t *step);
 /* calculate_photon_energy only (src/mesh.h:237-323) for the coming step */
 int orc_sim_prepare(orc_sim *sim);
 

@@ -1,0 +1,71 @@
+"""The product end to end: XML deck -> host driver -> HIP engine, against the
+reference executable's own printed per-step diagnostics
+(tests/golden/reference_runs.json; src/imc_state.h:264-284).
+
+No oracle in the loop: photon counts and census sizes must be identical at
+every step, energies must agree to the 6 printed digits, and conservation
+must close to round-off.
+"""
+import json
+import os
+
+import pytest
+
+from branson_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _close6(a, b):
+    return abs(a - b) <= 5.01e-6 * abs(b) if b != 0.0 else abs(a) < 1e-300
+
+
+@pytest.mark.parametrize("algo,prec", [(capi.ALGO_HISTORY, capi.PRECISION_FP64),
+                                       (capi.ALGO_PERSISTENT, capi.PRECISION_FP64)])
+@pytest.mark.parametrize("name", ["marshak_slab_100", "cube_40"])
+def test_driver_reproduces_reference_run(name, algo, prec):
+    run = json.load(open(os.path.join(GOLDEN, "reference_runs.json")))["runs"][name]
+    with capi.Driver(os.path.join(GOLDEN, run["deck"]), algorithm=algo, precision=prec) as d:
+        reps = d.run()
+    assert len(reps) == len(run["steps"])
+    for r, g in zip(reps, run["steps"]):
+        print(name, r["step"], r["transported"], r["census_size"], r["absorbed_E"],
+              r["rad_conservation"], flush=True)
+        assert r["step"] == g["step"]
+        assert r["transported"] == g["transported"]
+        assert r["census_size"] == g["census_size"]
+        for k in ("emission_E", "source_E", "absorbed_E", "exit_E", "pre_census_E",
+                  "post_census_E", "pre_mat_E", "post_mat_E"):
+            assert _close6(r[k], g[k]), (name, r["step"], k, r[k], g[k])
+        scale = max(r["post_mat_E"], r["absorbed_E"] + r["exit_E"] + r["post_census_E"])
+        assert abs(r["rad_conservation"]) < 1e-12 * scale
+        assert abs(r["mat_conservation"]) < 1e-12 * scale
+
+
+def test_driver_phases_and_state_errors():
+    with capi.Driver(os.path.join(GOLDEN, "decks", "cube_40.xml"), n_photons=50_000) as d:
+        with pytest.raises(capi.BhError):
+            d.transport()  # before prepare
+        d.prepare()
+        with pytest.raises(capi.BhError):
+            d.prepare()  # twice
+        d.transport()
+        rep = d.finish()
+        assert rep["transported"] == 64000  # max(1, int(5e4/64000)) per cell
+        assert d.finished()
+        with pytest.raises(capi.BhError):
+            d.prepare()  # after the last step
+
+
+def test_driver_fp32_tracking_runs_hohlraum_30_groups_small():
+    deck = os.path.join(os.path.dirname(GOLDEN), "..", "branson_amd", "decks", "hohlraum_64.xml")
+    with capi.Driver(deck, n_groups=30, n_photons=2_000_000, t_stop=0.002,
+                     precision=capi.PRECISION_FP32_TRACK,
+                     algorithm=capi.ALGO_PERSISTENT) as d:
+        reps = d.run()
+    for r in reps:
+        scale = r["source_E"] + r["emission_E"] + r["pre_census_E"]
+        assert abs(r["rad_conservation"]) < 1e-11 * scale
+        assert r["n_segments"] > r["transported"]

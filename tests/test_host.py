@@ -1,0 +1,107 @@
+"""Host side (include/branson_host.h) against the oracle, on the CPU.
+
+The deck reader, mesh numbering (src/proto_mesh.h:106-218) and step physics
+(Mesh::calculate_photon_energy, src/mesh.h:237-323) of the product are
+compared bit for bit with the oracle's restatement, itself pinned to the
+reference executable's printed energies (test_oracle_transport.py).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from branson_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN_DECKS = os.path.join(ROOT, "tests", "golden", "decks")
+PKG_DECKS = os.path.join(ROOT, "branson_amd", "decks")
+
+DECKS = [os.path.join(GOLDEN_DECKS, "marshak_slab_100.xml"),
+         os.path.join(GOLDEN_DECKS, "cube_40.xml"),
+         os.path.join(PKG_DECKS, "cube_40_throughput.xml"),
+         os.path.join(PKG_DECKS, "hohlraum_64.xml")]
+
+
+def test_host_symbols_exported():
+    L = capi.lib()
+    src = open(os.path.join(ROOT, "include", "branson_host.h")).read()
+    import re
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    declared = sorted(set(re.findall(r"\b(bh_driver_[a-z0-9_]+)\s*\(", src)))
+    assert declared == sorted(capi.HOST_EXPORTS)
+    for n in declared:
+        assert hasattr(L, n), n
+
+
+@pytest.mark.parametrize("deck", DECKS, ids=os.path.basename)
+def test_mesh_matches_oracle(deck):
+    with capi.Driver(deck) as d:
+        nodes, bc, nxt, reg = d.mesh()
+    sim = oracle.Sim(oracle.Deck.from_xml(deck))
+    o_nodes, o_bc, o_nxt, o_reg = sim.mesh()
+    assert np.array_equal(nodes, o_nodes)
+    assert np.array_equal(bc, o_bc)
+    assert np.array_equal(nxt, o_nxt)
+    assert np.array_equal(reg, o_reg)
+
+
+@pytest.mark.parametrize("deck,G", [(DECKS[0], 1), (DECKS[1], 1), (DECKS[3], 30)],
+                         ids=["marshak", "cube", "hohlraum30"])
+def test_step_physics_matches_oracle(deck, G):
+    with capi.Driver(deck, n_groups=G) as d:
+        d.calculate_physics()
+        ph = d.physics()
+    od = oracle.Deck.from_xml(deck, n_groups=G)
+    sim = oracle.Sim(od)
+    sim.prepare()
+    oph = sim.physics()
+    for k in ("sigma_a", "sigma_s", "fleck", "emission_E", "source_E", "census_E",
+              "source_face"):
+        assert np.array_equal(ph[k], oph[k]), k
+    assert ph["total_E"] == oph["total_E"]
+
+
+def test_hohlraum_geometry():
+    with capi.Driver(DECKS[3], n_groups=30) as d:
+        nodes, bc, nxt, reg = d.mesh()
+        d.calculate_physics()
+        ph = d.physics()
+    assert len(nodes) == 64 ** 3
+    # cell widths 0.01 cm, division ends snapped to the next start
+    w = nodes[:, 1] - nodes[:, 0]
+    assert np.allclose(w, 0.01, rtol=1e-12)
+    # only the centre division (56^3 cells) is fill
+    assert (reg == 1).sum() == 56 ** 3
+    # the source face is z = 0 for every bottom cell
+    bottom = nodes[:, 4] == 0.0
+    assert np.all(ph["source_face"][bottom] == 4) and np.all(ph["source_face"][~bottom] == -1)
+    assert np.all(ph["source_E"][bottom] > 0)
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda s: s.replace("<bc_right>VACUUM</bc_right>", "<bc_right>MIRROR</bc_right>"),
+     "Boundary type"),
+    (lambda s: s.replace("</regions>", "").replace("<regions>", ""), "regions"),
+    (lambda s: s.replace("<region_ID>1</region_ID>", "<region_ID>7</region_ID>"), "region"),
+    (lambda s: s.replace("</prototype>", ""), "missing"),
+])
+def test_bad_decks_are_rejected(tmp_path, mutate, msg):
+    src = open(DECKS[0]).read()
+    p = tmp_path / "bad.xml"
+    p.write_text(mutate(src))
+    with pytest.raises(capi.BhError) as e:
+        capi.Driver(str(p))
+    assert msg.lower() in str(e.value).lower()
+
+
+def test_missing_file():
+    with pytest.raises(capi.BhError):
+        capi.Driver("/nonexistent/deck.xml")
+
+
+def test_overrides_and_clock():
+    with capi.Driver(DECKS[1], n_photons=12345, t_stop=0.002) as d:
+        assert d.n_photons == 12345
+        assert d.seed == 1234
+        assert not d.finished()
