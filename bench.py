@@ -42,14 +42,21 @@ BYTES = {"fp64": dict(hist=194, seg=48, cross=8, flush=32),
          "fp32": dict(hist=138, seg=36, cross=8, flush=32)}
 
 
+N_PHOTONS = 10_000_000  # the deck's photons per step
+
+
 def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False):
-    """Warm up on one driver, then time `steps` steps on a fresh one."""
+    """Warm up on one driver, then time `steps` steps on a fresh one.  Photon
+    buffers are reserved up front (device memory is set-up, not step work):
+    the census grows to ~3.7x the photons per step over five steps."""
+    reserve = N_PHOTONS * (1 + min(steps, 8))
     if warmup:
         with capi.Driver(DECK, precision=precision, algorithm=algorithm,
                          t_stop=DT * warmup) as w:
             w.run()
     out = dict(hist=0, segs=0, flushes=0, dev_s=0.0, k_ms=[], k_bytes=[], step1=None)
-    with capi.Driver(DECK, precision=precision, algorithm=algorithm, t_stop=DT * steps) as d:
+    with capi.Driver(DECK, precision=precision, algorithm=algorithm, t_stop=DT * steps,
+                     reserve_photons=reserve) as d:
         eng = capi.lib()
         h = eng.bh_driver_engine(d._d)
         for k in range(steps):
@@ -120,6 +127,12 @@ def main(argv=None):
     from branson_amd import capi
     import numpy as np
 
+    # streaming-copy probe: the HBM rate this device reaches with a plain
+    # copy kernel (k_copy_probe, 1 GiB each way), also the calibration kernel
+    # for the FETCH_SIZE / WRITE_SIZE counters of the profiles
+    with capi.Engine(0) as e:
+        copy_gbps = e.copy_probe(1 << 30, 20)
+
     plain = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_HISTORY,
                       keep_step1=True)
     value = plain["hist"] / plain["dev_s"]
@@ -173,7 +186,8 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "k_transport_history<*, double>",
-                     "kernel_ms": kms, "algorithmic_bytes": bytes_launch},
+                     "kernel_ms": kms, "algorithmic_bytes": bytes_launch,
+                     "copy_probe_GBps": copy_gbps},
         "cpu_baseline": cpu,
         "chosen": chosen,
     }

@@ -331,12 +331,13 @@ class Driver:
     def __init__(self, xml_path: str, device: int = 0, n_groups: int = 1, rank: int = 0,
                  n_ranks: int = 1, n_photons: int = 0, t_stop: float = 0.0,
                  precision: int = PRECISION_FP64, algorithm: int = ALGO_HISTORY,
-                 tally_mode: int = 0, blocks_per_cu: int = 0, refill_threshold: int = 0):
+                 tally_mode: int = 0, blocks_per_cu: int = 0, refill_threshold: int = 0,
+                 reserve_photons: int = 0):
         L = _host_lib()
         self._d = C.c_void_p()
         cfg = DriverConfig(Options(precision, algorithm, 0, blocks_per_cu, tally_mode,
-                                   refill_threshold, 0), device, n_groups, rank, n_ranks,
-                           n_photons, t_stop)
+                                   refill_threshold, reserve_photons), device, n_groups,
+                           rank, n_ranks, n_photons, t_stop)
         rc = L.bh_driver_create(os.fsencode(xml_path), C.byref(cfg), C.byref(self._d))
         if rc != BH_OK:
             msg = L.bh_driver_last_error(self._d).decode()

@@ -28,34 +28,62 @@ constexpr uint32_t kMaxSegments = 1u << 26;
 // ---------------------------------------------------------------------------
 // Threefry2x64-20 (Salmon et al., SC'11; vendored by the reference as
 // src/random123/threefry.h:190-297).  Rotation table R_64x2 and the Skein key
-// schedule parity are the published constants; the 20 rounds unroll so every
-// rotation is an immediate funnel shift (two v_alignbit_b32 per rotate).
+// schedule parity are the published constants.  A 64-bit word lives in two
+// VGPRs; a rotate by a constant N is two v_alignbit_b32 funnel shifts (none
+// for N = 32, a register swap), an add is one v_lshl_add_u64.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned n) {
-  return (x << n) | (x >> (64u - n));
+template <unsigned N>
+__device__ __forceinline__ uint64_t rotl64c(uint64_t x) {
+  static_assert(N > 0 && N < 64, "rotation");
+  const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  uint32_t nh, nl;
+  if constexpr (N == 32) {
+    nh = lo;
+    nl = hi;
+  } else if constexpr (N < 32) {  // alignbit(a, b, s) = ({a, b} >> s)[31:0]
+    nh = __builtin_amdgcn_alignbit(hi, lo, 32u - N);
+    nl = __builtin_amdgcn_alignbit(lo, hi, 32u - N);
+  } else {
+    nh = __builtin_amdgcn_alignbit(lo, hi, 64u - N);
+    nl = __builtin_amdgcn_alignbit(hi, lo, 64u - N);
+  }
+  return ((uint64_t)nh << 32) | nl;
 }
 
 struct Threefry {
   static constexpr uint64_t kParity = 0x1BD11BDAA9FC1A22ULL;
 
-  // Full two-word output.
+  template <unsigned N>
+  __device__ __forceinline__ static void mix(uint64_t& x0, uint64_t& x1) {
+    x0 += x1;
+    x1 = rotl64c<N>(x1);
+    x1 ^= x0;
+  }
+  // four rounds with rotation constants R[4q .. 4q+3] then key injection s
+  template <unsigned A, unsigned B, unsigned C, unsigned D>
+  __device__ __forceinline__ static void quad(uint64_t& x0, uint64_t& x1, uint64_t ka,
+                                              uint64_t kb, uint64_t s) {
+    mix<A>(x0, x1);
+    mix<B>(x0, x1);
+    mix<C>(x0, x1);
+    mix<D>(x0, x1);
+    x0 += ka;
+    x1 += kb + s;
+  }
+
+  // Full two-word output of Threefry2x64 with 20 rounds.
   __device__ __forceinline__ static void block(uint64_t c0, uint64_t c1, uint64_t k0,
                                                uint64_t k1, uint64_t& o0, uint64_t& o1) {
-    constexpr unsigned R[8] = {16, 42, 12, 31, 16, 32, 24, 21};
-    const uint64_t ks[3] = {k0, k1, kParity ^ k0 ^ k1};
-    uint64_t x0 = c0 + ks[0];
-    uint64_t x1 = c1 + ks[1];
-#pragma unroll
-    for (unsigned r = 0; r < 20; ++r) {
-      x0 += x1;
-      x1 = rotl64(x1, R[r & 7u]);
-      x1 ^= x0;
-      if ((r & 3u) == 3u) {
-        const unsigned s = (r + 1u) >> 2;
-        x0 += ks[s % 3u];
-        x1 += ks[(s + 1u) % 3u] + (uint64_t)s;
-      }
-    }
+    const uint64_t k2 = kParity ^ k0 ^ k1;
+    uint64_t x0 = c0 + k0;
+    uint64_t x1 = c1 + k1;
+    // R_64x2 = {16, 42, 12, 31, 16, 32, 24, 21}; injection s uses
+    // ks[s % 3] and ks[(s + 1) % 3] + s
+    quad<16, 42, 12, 31>(x0, x1, k1, k2, 1);
+    quad<16, 32, 24, 21>(x0, x1, k2, k0, 2);
+    quad<16, 42, 12, 31>(x0, x1, k0, k1, 3);
+    quad<16, 32, 24, 21>(x0, x1, k1, k2, 4);
+    quad<16, 42, 12, 31>(x0, x1, k2, k0, 5);
     o0 = x0;
     o1 = x1;
   }

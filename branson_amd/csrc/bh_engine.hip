@@ -89,6 +89,7 @@ struct bh_handle {
   DArr<double2> sigma;
   DArr<uint32_t> next;
   bool has_mesh = false;
+  bool gray_sigma_a = false;  // sigma_a of every cell equal in all groups
 
   // per-cell source energies
   DArr<double> em, src, cen;
@@ -171,7 +172,9 @@ void release(DArr<T>& a) {
 // Grows a photon store to hold `need`, keeping the first `keep` photons.
 int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
   if (need <= s.cap && s.slab) return BH_OK;
-  uint64_t cap = std::max<uint64_t>(need + need / 8, 1024);
+  // grow by half again: a census that grows step after step reallocates
+  // rarely (each reallocation is a device-wide synchronising hipMalloc)
+  uint64_t cap = std::max<uint64_t>(need + need / 2, 1024);
   void* slab = nullptr;
   if (hipMalloc(&slab, PhotonStore::bytes_for(cap)) != hipSuccess)
     return fail(h, BH_ERR_ALLOC, "device allocation for %llu photons failed",
@@ -238,20 +241,30 @@ int check_cells(bh_handle* h, const bh_cells* c) {
 }
 
 // Runs `launch` with the tally sink chosen for this mesh: block-private LDS
-// tallies when both arrays fit, else global atomics.
+// tallies when both arrays fit, else wave-buffered global atomics -- one per
+// flush when every cell's absorption opacity is group-independent (then
+// k_tally_finalize completes the tallies), two otherwise.
 template <class Launch>
 int with_tally(bh_handle* h, bool lds, Launch&& launch) {
   if (lds) {
-    LdsTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr};
+    LdsTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
     return launch(t, (size_t)2 * h->n_cells * sizeof(double));
   }
-  GlobalTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr};
+  if (h->gray_sigma_a) {
+    GlobalTally<false> t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
+    TRY(launch(t, (size_t)0));
+    k_tally_finalize<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(
+        h->geo.p, h->sigma.p, h->G, h->n_cells, h->abs_E.p, h->track_E.p);
+    HIPCHK(h, hipGetLastError());
+    return BH_OK;
+  }
+  GlobalTally<true> t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
   return launch(t, (size_t)0);
 }
 
 bool use_lds_tally(const bh_handle* h, bool persistent) {
   const size_t bytes = (size_t)2 * h->n_cells * sizeof(double);
-  if (h->opt.tally_mode == 1) return false;
+  if (h->opt.tally_mode == 1 || h->opt.tally_mode == 3) return false;
   if (h->opt.tally_mode == 2) return bytes <= h->lds_budget;
   return bytes <= (persistent ? h->lds_budget : std::min<size_t>(h->lds_budget, 16 * 1024));
 }
@@ -333,7 +346,7 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
     return r;
   }
   if (h->opt.precision < 0 || h->opt.precision > 1 || h->opt.algorithm < 0 ||
-      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 2) {
+      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 3) {
     int r = fail(h, BH_ERR_ARG, "invalid bh_options");
     *out = h;
     return r;
@@ -448,6 +461,14 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
   h->n_cells = n;
   h->G = G;
   h->has_mesh = true;
+  h->gray_sigma_a = true;
+  for (uint32_t i = 0; i < n && h->gray_sigma_a; ++i)
+    for (uint32_t g = 1; g < G; ++g)
+      if (c->sigma_a[(size_t)i * G + g] != c->sigma_a[(size_t)i * G]) {
+        h->gray_sigma_a = false;
+        break;
+      }
+  if (h->opt.tally_mode == 3) h->gray_sigma_a = false;  // force two tallies per flush
   return BH_OK;
 }
 

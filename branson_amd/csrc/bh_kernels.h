@@ -257,32 +257,129 @@ __global__ __launch_bounds__(256) void k_source_fill(
 }
 
 // ---------------------------------------------------------------------------
-// Tally sinks.  A flush adds a photon's accumulated (absorbed, track) energy
-// of one cell visit.  GlobalTally: native global_atomic_add_f64 at L2.
-// LdsTally: block-private tallies in LDS (ds_add_f64), added to HBM once per
-// block -- for meshes small enough that every block can hold them, where
-// global atomics would serialise on a few hot cells.
+// Tally sinks.  A flush adds one photon's accumulated (absorbed, track)
+// energy of one cell visit; push() is called by every lane of a wave at the
+// same program point, with has = false for lanes that did not flush.
+//
+// GlobalTally: a 128-entry ring of flush records per wave in LDS.  Lanes
+// append their records (ballot + mbcnt); whenever 64 are queued the whole
+// wave drains them with one full-wave global_atomic_add_f64 per tally array.
+// On gfx950 vmcnt counts atomics in order with loads, so an atomic issued by
+// a few divergent lanes puts a full atomic round trip in front of the next
+// opacity fetch; batching turns ~10 such stalls into one and fills all 64
+// lanes of every atomic instruction.
+//
+// LdsTally: block-private tallies of the whole mesh in LDS (ds_add_f64),
+// added to HBM once per block -- for meshes small enough that every block
+// can hold them, where global atomics would serialise on a few hot cells.
 // ---------------------------------------------------------------------------
+struct alignas(8) FlushRec {
+  double a;
+  double t;
+  uint32_t c;
+  uint32_t pad;
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A flush: one cell visit of one photon.  cont = implicit-capture deposit,
+// track = sum of deposit/(sigma_a f) over the visit, kill = the weight
+// deposited when the photon is killed (0 otherwise).  The reference adds the
+// kill weight to the visit's absorbed energy (src/history_based_transport.h:
+// 85-88), so abs = cont + kill.
+struct Flush {
+  bool has;
+  uint32_t c;
+  double cont, track, kill;
+};
+
+// kTrack = true: two tallies per flush, abs_E += cont + kill and
+// track_E += track (any opacities).
+// kTrack = false: for cells whose absorption opacity is the same in every
+// group (Branson's faux multigroup, Cell::set_op_a) the track tally of a
+// visit is cont/(sigma_a f), so only cont is tallied per flush (into abs_E)
+// and kill weights separately (into track_E, rare); k_tally_finalize turns
+// them into abs = cont + kill, track = cont/(sigma_a f).  Half the atomics
+// and no per-segment division.
+template <bool kTrackT>
 struct GlobalTally {
   static constexpr bool kLds = false;
+  static constexpr bool kTrack = kTrackT;
+  static constexpr uint32_t kRing = 128;  // power of two, >= 2 * 64
   double* abs_E;
   double* track_E;
   double* s_abs;  // unused
   double* s_track;
-  __device__ __forceinline__ void init(uint32_t) {}
-  __device__ __forceinline__ void add(uint32_t c, double a, double t) {
-    unsafeAtomicAdd(&abs_E[c], a);
-    unsafeAtomicAdd(&track_E[c], t);
+  uint32_t head, count;  // wave-uniform ring state
+
+  __device__ __forceinline__ static FlushRec* ring() {
+    __shared__ FlushRec r[256 / kWave][kRing];
+    return r[threadIdx.x / kWave];
   }
-  __device__ __forceinline__ void finish(uint32_t) {}
+  __device__ __forceinline__ void init(uint32_t) {
+    head = 0;
+    count = 0;
+  }
+  __device__ __forceinline__ void drain_one(const FlushRec& e) {
+    if constexpr (kTrack) {
+      unsafeAtomicAdd(&abs_E[e.c], e.a);
+      unsafeAtomicAdd(&track_E[e.c], e.t);
+    } else {
+      unsafeAtomicAdd(&abs_E[e.c], e.a);
+      if (e.t != 0.0) unsafeAtomicAdd(&track_E[e.c], e.t);
+    }
+  }
+  __device__ __forceinline__ void push(const Flush& f) {
+    const uint64_t m = __ballot(f.has);
+    if (m == 0) return;
+    FlushRec* r = ring();
+    if (f.has) {
+      const uint32_t pos = (head + count + lane_rank(m)) & (kRing - 1);
+      r[pos].a = kTrack ? f.cont + f.kill : f.cont;
+      r[pos].t = kTrack ? f.track : f.kill;
+      r[pos].c = f.c;
+    }
+    count += (uint32_t)__popcll(m);
+    if (count >= kWave) {
+      wave_lds_sync();
+      drain_one(r[(head + (threadIdx.x & (kWave - 1))) & (kRing - 1)]);
+      head = (head + kWave) & (kRing - 1);
+      count -= kWave;
+      wave_lds_sync();
+    }
+  }
+  __device__ __forceinline__ void finish(uint32_t) {
+    wave_lds_sync();
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    if (lane < count) drain_one(ring()[(head + lane) & (kRing - 1)]);
+  }
 };
+
+// abs = cont + kill, track = cont / (sigma_a f) after a GlobalTally<false>
+// transport (sigma_a of group 0: every group holds the same value).
+__global__ void k_tally_finalize(const CellGeo* __restrict__ geo,
+                                 const double2* __restrict__ sigma, uint32_t G,
+                                 uint32_t n_cells, double* __restrict__ abs_E,
+                                 double* __restrict__ track_E) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const double cont = abs_E[c], kill = track_E[c];
+  abs_E[c] = cont + kill;
+  track_E[c] = cont / (sigma[(size_t)c * G].x * geo[c].fleck);
+}
 
 struct LdsTally {
   static constexpr bool kLds = true;
+  static constexpr bool kTrack = true;
   double* abs_E;  // global destination
   double* track_E;
   double* s_abs;  // dynamic LDS, n_cells each
   double* s_track;
+  uint32_t head, count;  // unused
   __device__ __forceinline__ void init(uint32_t n_cells) {
     for (uint32_t c = threadIdx.x; c < n_cells; c += blockDim.x) {
       s_abs[c] = 0.0;
@@ -290,9 +387,13 @@ struct LdsTally {
     }
     __syncthreads();
   }
-  __device__ __forceinline__ void add(uint32_t c, double a, double t) {
-    __hip_atomic_fetch_add(&s_abs[c], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(&s_track[c], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __device__ __forceinline__ void push(const Flush& f) {
+    if (f.has) {
+      __hip_atomic_fetch_add(&s_abs[f.c], f.cont + f.kill, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&s_track[f.c], f.track, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
   }
   __device__ __forceinline__ void finish(uint32_t n_cells) {
     __syncthreads();

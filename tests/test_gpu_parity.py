@@ -129,8 +129,12 @@ def _transport_both(d, n_user, engine_kw, n_threads=8):
     return ref, got, (a_o, t_o, s_o), (a_g, t_g, st)
 
 
+# tally_mode: 0 auto (LDS for the 100-cell slab, one global atomic per flush
+# for the cube), 1 global, 2 LDS where it fits, 3 two global atomics per flush
 @pytest.mark.parametrize("algo,tally", [(capi.ALGO_HISTORY, 0), (capi.ALGO_HISTORY, 1),
-                                        (capi.ALGO_PERSISTENT, 0), (capi.ALGO_PERSISTENT, 1)])
+                                        (capi.ALGO_HISTORY, 3), (capi.ALGO_PERSISTENT, 0),
+                                        (capi.ALGO_PERSISTENT, 1), (capi.ALGO_PERSISTENT, 2),
+                                        (capi.ALGO_PERSISTENT, 3)])
 @pytest.mark.parametrize("deck,n_user", [("cube_40.xml", 300_000),
                                          ("marshak_slab_100.xml", 50_000)])
 def test_transport_photons_fp64_matches_oracle(deck, n_user, algo, tally):
@@ -368,6 +372,36 @@ def test_census_import_export_roundtrip_and_groups():
     census_o, st_o = oracle.post_process(clk["next_dt"], ref)
     assert st["n_census"] == st_o["n_census"]
     assert np.allclose(a_g, a_o, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT])
+def test_group_dependent_opacities(algo):
+    """Per-group sigma_a (allowed by the C-ABI although Branson's decks fill
+    every group alike): the re-emission group walk sees a non-uniform CDF
+    and the track tally must be summed per segment."""
+    d = _deck("cube_40.xml")
+    d.n_groups = 8
+    sim = oracle.Sim(d)
+    cells0, ph, clk = _step_inputs(sim)
+    rng = np.random.default_rng(5)
+    sa = cells0.sigma_a * rng.uniform(0.3, 3.0, size=(1, 8))
+    ss = cells0.sigma_s * rng.uniform(0.5, 2.0, size=(1, 8))
+    cells = oracle.Cells(cells0.nodes, cells0.bc, cells0.next, sa, ss, cells0.fleck)
+    photons = oracle.make_photons(cells, ph["emission_E"], ph["source_E"], ph["source_face"],
+                                  150_000, ph["total_E"], d.seed, 1, 0, clk["dt"])
+    ref = photons.copy()
+    a_o, t_o, s_o = oracle.transport(cells, ref, 8)
+    got = photons.copy()
+    with _engine(algorithm=algo) as e:
+        e.set_mesh_cells(cells)
+        a_g, t_g, st = e.transport_photons(got)
+    assert len(np.unique(ref["group"])) == 8
+    assert np.array_equal(got["descriptor"][:, 0], ref["descriptor"][:, 0])
+    assert np.array_equal(got["group"], ref["group"])
+    assert np.array_equal(got["rng"], ref["rng"])
+    assert st["n_segments"] == s_o
+    assert np.allclose(a_g, a_o, rtol=1e-12, atol=1e-300)
+    assert np.allclose(t_g, t_o, rtol=1e-12, atol=1e-300)
 
 
 # ---------------------------------------------------------------------------
