@@ -126,6 +126,7 @@ class _StepReport(C.Structure):
         ("rad_conservation", C.c_double),
         ("mat_conservation", C.c_double),
         ("transport_seconds", C.c_double),
+        ("source_seconds", C.c_double),
     ]
 
 

@@ -821,6 +821,7 @@ int orc_sim_step(orc_sim *s, int n_threads, orc_step_report *rep) {
                      s->sigma_a, s->sigma_s, s->fleck};
 
   s->pre_census_E = orc_list_E(s->census, s->n_census);
+  const double ts = orc_seconds();
   if (s->step == 1) {
     const uint64_t nc = orc_make_initial_census(&cells, s->census_E, s->n_user,
                                                 global_E, s->seed, 0, dt, NULL,
@@ -843,6 +844,7 @@ int orc_sim_step(orc_sim *s, int n_threads, orc_step_report *rep) {
   if (s->n_census)
     memcpy(all + n_new, s->census, sizeof(orc_photon) * s->n_census);
   rep->transported = n_all;
+  rep->source_seconds = orc_seconds() - ts;
 
   /* replicated_transport (src/replicated_transport.h:50-157) */
   const double next_dt = orc_next_dt(s);

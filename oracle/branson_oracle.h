@@ -161,7 +161,8 @@ typedef struct orc_step_report {
   double emission_E, source_E, absorbed_E, exit_E;
   double pre_census_E, post_census_E, pre_mat_E, post_mat_E;
   double rad_conservation, mat_conservation;
-  double transport_seconds;
+  double transport_seconds; /* transport + post-processing into the census */
+  double source_seconds;    /* initial census + make_photons + join         */
 } orc_step_report;
 
 orc_sim *orc_sim_create(const orc_deck *deck);
