@@ -37,7 +37,8 @@ EXPORTS = [
     "bh_set_source_energy", "bh_source", "bh_source_from_energy", "bh_transport",
     "bh_get_tallies", "bh_census_size", "bh_census_export", "bh_photons_export",
     "bh_census_import",
-    "bh_transport_photons", "bh_rng_draws", "bh_threefry", "bh_copy_probe",
+    "bh_transport_photons", "bh_adjoint_photons", "bh_rng_draws", "bh_threefry",
+    "bh_copy_probe",
 ]
 
 
@@ -50,7 +51,8 @@ class BhError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("algorithm", C.c_int32), ("block_size", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("tally_mode", C.c_int32),
-                ("refill_threshold", C.c_int32), ("reserve_photons", C.c_uint64)]
+                ("refill_threshold", C.c_int32), ("reserve_photons", C.c_uint64),
+                ("fp32_tol", C.c_double)]
 
 
 class Cells(C.Structure):
@@ -64,7 +66,8 @@ class StepStats(C.Structure):
                 ("n_transported", C.c_uint64), ("n_census", C.c_uint64),
                 ("n_exit", C.c_uint64), ("n_killed", C.c_uint64), ("n_segments", C.c_uint64),
                 ("n_flushes", C.c_uint64), ("transport_ms", C.c_double),
-                ("census_ms", C.c_double), ("source_ms", C.c_double)]
+                ("census_ms", C.c_double), ("source_ms", C.c_double),
+                ("n_replayed", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -100,6 +103,7 @@ def lib() -> C.CDLL:
     L.bh_census_import.argtypes = [P, P, U64]
     L.bh_photons_export.argtypes = [P, P, U64, P]
     L.bh_transport_photons.argtypes = [P, P, U64, P, P, P]
+    L.bh_adjoint_photons.argtypes = [P, P, U64, P, P, P, P]
     L.bh_rng_draws.argtypes = [P, U32, U64, U64, U32, P]
     L.bh_threefry.argtypes = [P, P, U64, P]
     L.bh_copy_probe.argtypes = [P, U64, I, P]
@@ -119,11 +123,11 @@ class Engine:
 
     def __init__(self, device: int = 0, precision: int = PRECISION_FP64,
                  algorithm: int = ALGO_HISTORY, tally_mode: int = 0, blocks_per_cu: int = 0,
-                 refill_threshold: int = 0, reserve_photons: int = 0):
+                 refill_threshold: int = 0, reserve_photons: int = 0, fp32_tol: float = 0.0):
         L = lib()
         self._h = C.c_void_p()
         self.opts = Options(precision, algorithm, 0, blocks_per_cu, tally_mode,
-                            refill_threshold, reserve_photons)
+                            refill_threshold, reserve_photons, fp32_tol)
         rc = L.bh_create(device, C.byref(self.opts), C.byref(self._h))
         if rc != BH_OK:
             msg = L.bh_last_error(self._h).decode() if self._h else "bh_create failed"
@@ -235,6 +239,19 @@ class Engine:
                                                _p(t), C.byref(st)))
         return a, t, st.as_dict()
 
+    def adjoint_photons(self, photons: np.ndarray, abs_adjoint):
+        """Path-replay adjoint: (dL/dsigma_a, dL/dfleck, stats) per cell for
+        photons in their pre-transport state and abs_adjoint = dL/d abs_E."""
+        assert photons.dtype == PHOTON_DTYPE and photons.flags.c_contiguous
+        g = np.ascontiguousarray(abs_adjoint, dtype=np.float64)
+        if g.shape != (self.n_cells,):
+            raise ValueError(f"abs_adjoint must have {self.n_cells} entries")
+        da, df = np.zeros(self.n_cells), np.zeros(self.n_cells)
+        st = StepStats()
+        self._check(lib().bh_adjoint_photons(self._h, _p(photons), len(photons), _p(g), _p(da),
+                                             _p(df), C.byref(st)))
+        return da, df, st.as_dict()
+
     # -- RNG probes ----------------------------------------------------------
     def rng_draws(self, seed, stream, n, ctr0=0):
         out = np.zeros(n)
@@ -284,7 +301,8 @@ class DriverReport(C.Structure):
                 ("rad_conservation", C.c_double), ("mat_conservation", C.c_double),
                 ("source_ms", C.c_double), ("transport_ms", C.c_double),
                 ("census_ms", C.c_double), ("prepare_ms", C.c_double),
-                ("device_phase_ms", C.c_double), ("finish_ms", C.c_double)]
+                ("device_phase_ms", C.c_double), ("finish_ms", C.c_double),
+                ("n_replayed", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -332,11 +350,11 @@ class Driver:
                  n_ranks: int = 1, n_photons: int = 0, t_stop: float = 0.0,
                  precision: int = PRECISION_FP64, algorithm: int = ALGO_HISTORY,
                  tally_mode: int = 0, blocks_per_cu: int = 0, refill_threshold: int = 0,
-                 reserve_photons: int = 0):
+                 reserve_photons: int = 0, fp32_tol: float = 0.0):
         L = _host_lib()
         self._d = C.c_void_p()
         cfg = DriverConfig(Options(precision, algorithm, 0, blocks_per_cu, tally_mode,
-                                   refill_threshold, reserve_photons), device, n_groups,
+                                   refill_threshold, reserve_photons, fp32_tol), device, n_groups,
                            rank, n_ranks, n_photons, t_stop)
         rc = L.bh_driver_create(os.fsencode(xml_path), C.byref(cfg), C.byref(self._d))
         if rc != BH_OK:

@@ -19,7 +19,10 @@ constexpr double kCutoff = 0.01;   // E/E0 below which a photon is killed
 
 enum : int { kReflect = 0, kVacuum = 1, kElement = 2, kSource = 3, kProcessor = 4 };
 enum : uint8_t { kExit = 0, kPass = 1, kCensus = 2, kScatter = 3, kKilled = 4, kBound = 5,
-                 kStuck = 255 };
+                 kReplay = 254, kStuck = 255 };
+// kReplay: internal to one transport call -- an FP32-tracked history whose
+// next decision was too close to call; it is replayed in FP64 before the
+// call returns, so the descriptor never reaches the census.
 
 // A photon whose history exceeds this many segments is stopped and flagged
 // (kStuck) so that no wave can spin forever on malformed input.

@@ -89,7 +89,8 @@ struct bh_handle {
   DArr<double2> sigma;
   DArr<uint32_t> next;
   bool has_mesh = false;
-  bool gray_sigma_a = false;  // sigma_a of every cell equal in all groups
+  bool gray_sigma_a = false;     // one tally atomic per flush (see with_tally)
+  bool uniform_sigma_a = false;  // sigma_a of every cell equal in all groups
 
   // per-cell source energies
   DArr<double> em, src, cen;
@@ -108,6 +109,10 @@ struct bh_handle {
   DArr<uint64_t> counts, off, tiles, tiles2;
   DArr<CensusPartial> parts;
   DArr<AosPhoton> aos;
+  DArr<double> adj_g, grad_sa, grad_f;  // adjoint: dL/d abs_E in, gradients out
+  DArr<uint64_t> rep_idx;    // FP32 form: histories handed to the FP64 replay
+  DArr<uint32_t> rep_first;
+  DArr<double> rep_E;
   uint64_t* d_total = nullptr;      // scan totals
   CensusPartial* d_ctot = nullptr;  // census totals
   DevCounters* d_ctrs = nullptr;
@@ -194,7 +199,8 @@ int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
 }
 
 DevCells dev_cells(const bh_handle* h) {
-  return DevCells{h->geo.p, h->sigma.p, h->next.p, h->n_cells, h->G};
+  const double tol = h->opt.fp32_tol > 0.0 ? h->opt.fp32_tol : 1e-6;
+  return DevCells{h->geo.p, h->sigma.p, h->next.p, h->n_cells, h->G, (float)tol};
 }
 
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)div_up(n, block); }
@@ -269,6 +275,17 @@ bool use_lds_tally(const bh_handle* h, bool persistent) {
   return bytes <= (persistent ? h->lds_budget : std::min<size_t>(h->lds_budget, 16 * 1024));
 }
 
+// FP64 replay of the histories the FP32 kernel stopped (list length on the
+// device, in h->d_ctrs->replays); same tally sink as the kernel before it.
+template <class T>
+int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const ReplayList& rl,
+                  T t, size_t shm) {
+  k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(cells, W, rl, h->ctr_hi,
+                                                                     t, h->d_ctrs);
+  HIPCHK(h, hipGetLastError());
+  return BH_OK;
+}
+
 // Transport of W[0, n) with the configured kernel; counters in h->d_ctrs.
 int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n) {
   if (n == 0) return BH_OK;
@@ -277,15 +294,27 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n) {
   const bool persistent = h->opt.algorithm == BH_ALGO_PERSISTENT;
   const bool lds = use_lds_tally(h, persistent);
   const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
+  ReplayList rl{nullptr, nullptr, nullptr};
+  if (fp32) {
+    // room for every photon of the launch, sized with the photon stores so
+    // that a growing census does not reallocate it every step
+    const size_t cap = std::max<size_t>(n, std::max(h->W.cap, h->C.cap));
+    TRY(ensure(h, h->rep_idx, cap));
+    TRY(ensure(h, h->rep_first, cap));
+    TRY(ensure(h, h->rep_E, cap));
+    rl = ReplayList{h->rep_idx.p, h->rep_first.p, h->rep_E.p};
+  }
   if (!persistent) {
     return with_tally(h, lds, [&](auto t, size_t shm) -> int {
       using T = decltype(t);
-      if (fp32)
+      if (fp32) {
         k_transport_history<T, float><<<grid_for(n, block), block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs);
-      else
-        k_transport_history<T, double><<<grid_for(n, block), block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs);
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+        HIPCHK(h, hipGetLastError());
+        return launch_replay(h, cells, W, rl, t, shm);
+      }
+      k_transport_history<T, double><<<grid_for(n, block), block, shm, h->stream>>>(
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
       HIPCHK(h, hipGetLastError());
       return BH_OK;
     });
@@ -298,12 +327,14 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n) {
                                       sizeof(PersistentQueue) / 8);
   return with_tally(h, lds, [&](auto t, size_t shm) -> int {
     using T = decltype(t);
-    if (fp32)
+    if (fp32) {
       k_transport_persistent<T, float><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill);
-    else
-      k_transport_persistent<T, double><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill);
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, rl);
+      HIPCHK(h, hipGetLastError());
+      return launch_replay(h, cells, W, rl, t, shm);
+    }
+    k_transport_persistent<T, double><<<grid, block, shm, h->stream>>>(
+        cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, rl);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   });
@@ -346,7 +377,8 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
     return r;
   }
   if (h->opt.precision < 0 || h->opt.precision > 1 || h->opt.algorithm < 0 ||
-      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 3) {
+      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 3 ||
+      !(h->opt.fp32_tol >= 0.0 && h->opt.fp32_tol < 1.0)) {
     int r = fail(h, BH_ERR_ARG, "invalid bh_options");
     *out = h;
     return r;
@@ -400,6 +432,12 @@ void bh_destroy(bh_handle* h) {
   release(h->tiles2);
   release(h->parts);
   release(h->aos);
+  release(h->adj_g);
+  release(h->grad_sa);
+  release(h->grad_f);
+  release(h->rep_idx);
+  release(h->rep_first);
+  release(h->rep_E);
   void* ptrs[] = {h->W.slab, h->C.slab, h->d_total, h->d_ctot, h->d_ctrs, h->d_queue};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -457,18 +495,21 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (h->has_mesh && (n != h->n_cells || G != h->G)) {
     h->has_energy = false;  // energies were sized for the old mesh
+    h->n_cen = 0;           // and a resident census indexes the old cells
+    h->n_new = 0;
   }
   h->n_cells = n;
   h->G = G;
   h->has_mesh = true;
-  h->gray_sigma_a = true;
-  for (uint32_t i = 0; i < n && h->gray_sigma_a; ++i)
+  h->uniform_sigma_a = true;
+  for (uint32_t i = 0; i < n && h->uniform_sigma_a; ++i)
     for (uint32_t g = 1; g < G; ++g)
       if (c->sigma_a[(size_t)i * G + g] != c->sigma_a[(size_t)i * G]) {
-        h->gray_sigma_a = false;
+        h->uniform_sigma_a = false;
         break;
       }
-  if (h->opt.tally_mode == 3) h->gray_sigma_a = false;  // force two tallies per flush
+  // tally_mode 3 forces two tallies per flush
+  h->gray_sigma_a = h->uniform_sigma_a && h->opt.tally_mode != 3;
   return BH_OK;
 }
 
@@ -622,6 +663,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
     stats->n_killed = tot.n_killed;
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
+    stats->n_replayed = ctrs.replays;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     stats->census_ms = elapsed_ms(h->ev[1], h->ev[2]);
     stats->source_ms = h->last_source_ms;
@@ -770,6 +812,7 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     stats->n_transported = n;
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
+    stats->n_replayed = ctrs.replays;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     for (uint64_t i = 0; i < n; ++i) {
       const uint8_t d = photons[i].descriptor[0];
@@ -783,6 +826,66 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
         stats->n_killed++;
       }
     }
+  }
+  if (ctrs.stuck)
+    return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
+                ctrs.stuck, kMaxSegments);
+  return BH_OK;
+}
+
+int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
+                       const double* abs_adjoint, double* d_sigma_a, double* d_fleck,
+                       bh_step_stats* stats) {
+  if (!h || !h->stream || (n && !photons) || !abs_adjoint || !d_sigma_a || !d_fleck)
+    return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_adjoint_photons before bh_set_mesh");
+  if (!h->uniform_sigma_a)
+    return fail(h, BH_ERR_ARG,
+                "the adjoint needs sigma_a equal in every group of a cell (the group "
+                "walk's probabilities must not depend on it)");
+  HIPCHK(h, hipSetDevice(h->device));
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (n == 0) return BH_OK;
+  const uint64_t saved_hi = h->ctr_hi;
+  TRY(upload_aos(h, photons, n, h->C, 0));
+  const uint32_t nc = h->n_cells;
+  TRY(ensure(h, h->adj_g, nc));
+  TRY(ensure(h, h->grad_sa, nc));
+  TRY(ensure(h, h->grad_f, nc));
+  HIPCHK(h, hipMemcpyAsync(h->adj_g.p, abs_adjoint, nc * sizeof(double), hipMemcpyHostToDevice,
+                           h->stream));
+  k_fill_f64<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->grad_sa.p, nc, 0.0);
+  k_fill_f64<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->grad_f.p, nc, 0.0);
+  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_ctrs),
+                                      sizeof(DevCounters) / 8);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+  GlobalTally<true> sink{h->grad_sa.p, h->grad_f.p, nullptr, nullptr, 0, 0};
+  k_adjoint<GlobalTally<true>><<<grid_for(n, 256), 256, 0, h->stream>>>(
+      dev_cells(h), h->C.soa, n, h->ctr_hi, h->adj_g.p, sink, h->d_ctrs);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+  std::vector<double> a(nc), f(nc);
+  HIPCHK(h, hipMemcpyAsync(a.data(), h->grad_sa.p, nc * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(f.data(), h->grad_f.p, nc * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  char* hp = static_cast<char*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp + 256, h->d_ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ctr_hi = h->n_cen ? saved_hi : h->ctr_hi;
+  DevCounters ctrs;
+  std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  for (uint32_t c = 0; c < nc; ++c) {
+    d_sigma_a[c] += a[c];
+    d_fleck[c] += f[c];
+  }
+  if (stats) {
+    stats->n_transported = n;
+    stats->n_segments = ctrs.segments;
+    stats->n_flushes = ctrs.flushes;
+    stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
   }
   if (ctrs.stuck)
     return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",

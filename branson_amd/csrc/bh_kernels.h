@@ -29,13 +29,23 @@ struct DevCells {
   const uint32_t* next;   // [n_cells * 6]
   uint32_t n_cells;
   uint32_t G;
+  float fp32_tol;         // FP32 tracking: largest relative error bound of a deposit
 };
 
 struct DevCounters {
   unsigned long long segments;
   unsigned long long flushes;
   unsigned long long stuck;
-  unsigned long long pad;
+  unsigned long long replays;  // histories handed from FP32 tracking to FP64 replay
+};
+
+// Histories the FP32-tracking kernels hand over for FP64 replay: the photon
+// index (its initial state is still in place), the first segment whose
+// deposits the replay tallies, and the FP32 weight at that point.
+struct ReplayList {
+  uint64_t* idx;
+  uint32_t* first_seg;
+  double* E;
 };
 
 struct CensusPartial {  // per block of the census compaction

@@ -19,9 +19,19 @@
 //
 // Precision forms:
 //   FP64 (Real = double): statement-for-statement replay of the reference;
-//   FP32 tracking (Real = float): position, direction, path lengths and the
-//     decay exponent in FP32; energy weights, tallies and the re-emission
-//     test stay FP64 so energy conservation is unchanged.
+//   FP32 tracking (Real = float): position (in the frame of the photon's
+//     cell: offsets from its low corner, so an ulp is 2^-24 of a cell width,
+//     not of the domain), direction, path lengths and the decay exponent in
+//     FP32; energy weights, tallies, the re-emission test and the group walk
+//     stay FP64.  Each history carries running bounds on its FP32 position
+//     error (across and along the ray), life error and the relative error
+//     of its weight.  A decision the bounds cannot separate -- which of scatter / face / census
+//     comes first, which face, whether E/E0 falls below the cutoff -- stops
+//     the history before that segment is applied: the deposits of the
+//     segments already decided are tallied, and k_transport_replay re-runs
+//     the history in FP64 from its initial state (still in place), tallying
+//     from that segment on.  Every event therefore matches the FP64 form; the
+//     tallies differ only by FP32 rounding of the decided segments.
 #pragma once
 
 #include "bh_kernels.h"
@@ -53,40 +63,76 @@ struct Live {
   double t_abs, t_track;
   Rng rng;
   uint32_t cell, group, sc, segs;
+  uint32_t first_seg;  // FP64 replay: deposits of earlier segments are muted
+  uint32_t frame;      // FP32 form: the cell whose low corner x, y, z are from
+  // FP32 form, error bounds against exact arithmetic from the same start:
+  // position error perpendicular to the ray (eP) and along it (eT), life
+  // (eL), all in cm, and the relative error of the weight (eE)
+  float eP, eT, eL, eE;
+  double E_handoff;    // FP64 replay: the FP32 weight when FP32 stopped
   uint8_t desc;
   bool pending;  // an effective scatter awaits its draws
 };
 
+// FP32 rounding constants of the error bounds: kU is the unit roundoff (one
+// correctly rounded operation), kDir bounds the error of each FP32
+// direction component, kE covers a few ulps (logf, expm1f, expf).
+constexpr float kU = 0x1p-24f;
+constexpr float kDir = 0x1p-22f;
+constexpr float kE = 0x1p-21f;
+
 template <typename R>
-__device__ __forceinline__ void load_live(const PhotonSoA& W, uint64_t i, uint64_t ctr_hi,
-                                          Live<R>& L) {
-  L.x = (R)W.x[i];
-  L.y = (R)W.y[i];
-  L.z = (R)W.z[i];
+__device__ __forceinline__ void load_live(const DevCells& cells, const PhotonSoA& W, uint64_t i,
+                                          uint64_t ctr_hi, Live<R>& L) {
+  L.cell = W.cell[i];
+  if constexpr (sizeof(R) == 4) {
+    const CellGeo* g = &cells.geo[L.cell];
+    L.frame = L.cell;
+    L.x = (float)(W.x[i] - g->nodes[0]);
+    L.y = (float)(W.y[i] - g->nodes[2]);
+    L.z = (float)(W.z[i] - g->nodes[4]);
+    L.life = (float)W.life[i];
+    L.eP = kU * (fabsf(L.x) + fabsf(L.y) + fabsf(L.z));
+    L.eT = L.eP;
+    L.eL = kU * fabsf(L.life);
+    L.eE = 0.0f;
+  } else {
+    L.x = W.x[i];
+    L.y = W.y[i];
+    L.z = W.z[i];
+    L.life = W.life[i];
+  }
   L.u = (R)W.u[i];
   L.v = (R)W.v[i];
   L.w = (R)W.w[i];
-  L.life = (R)W.life[i];
   L.E = W.E[i];
   L.E0 = W.E0[i];
   L.rng.ctr = W.ctr[i];
   L.rng.stream = W.stream[i];
   L.rng.ctr_hi = ctr_hi;
-  L.cell = W.cell[i];
   L.group = W.group[i];
   L.desc = W.desc[i];
   L.sc = 0;  // surface_cross starts at 0 for every photon (:43)
   L.segs = 0;
   L.t_abs = 0.0;
   L.t_track = 0.0;
+  L.first_seg = 0;
   L.pending = false;
 }
 
 template <typename R>
-__device__ __forceinline__ void store_live(const PhotonSoA& W, uint64_t i, const Live<R>& L) {
-  W.x[i] = (double)L.x;
-  W.y[i] = (double)L.y;
-  W.z[i] = (double)L.z;
+__device__ __forceinline__ void store_live(const DevCells& cells, const PhotonSoA& W, uint64_t i,
+                                           const Live<R>& L) {
+  if constexpr (sizeof(R) == 4) {
+    const CellGeo* g = &cells.geo[L.frame];
+    W.x[i] = g->nodes[0] + (double)L.x;
+    W.y[i] = g->nodes[2] + (double)L.y;
+    W.z[i] = g->nodes[4] + (double)L.z;
+  } else {
+    W.x[i] = L.x;
+    W.y[i] = L.y;
+    W.z[i] = L.z;
+  }
   W.u[i] = (double)L.u;
   W.v[i] = (double)L.v;
   W.w[i] = (double)L.w;
@@ -112,13 +158,48 @@ __device__ __forceinline__ void flush(Flush& fo, const Live<R>& L) {
 
 // Boundary event shared by both precisions (:102-131).  Returns true when
 // the history ends.
+// Moving to another cell of this mesh: FP64 positions are global; FP32
+// positions move to the new cell's frame (one FP32 rounding of the shift on
+// the crossing axis; zero shift on the other axes of a rectilinear mesh).
+// A PROCESSOR face ends the history with the neighbour's index, which may
+// not be a cell of this mesh, so there the frame stays with the old cell.
+__device__ __forceinline__ void enter_cell(const DevCells&, const CellGeo*, uint32_t nc,
+                                           Live<double>& L) {
+  L.cell = nc;
+}
+__device__ __forceinline__ void enter_cell(const DevCells& cells, const CellGeo* from,
+                                           uint32_t nc, Live<float>& L) {
+  const CellGeo* to = &cells.geo[nc];
+  const float sx = (float)(to->nodes[0] - from->nodes[0]);
+  const float sy = (float)(to->nodes[2] - from->nodes[2]);
+  const float sz = (float)(to->nodes[4] - from->nodes[4]);
+  L.x -= sx;
+  L.y -= sy;
+  L.z -= sz;
+  const float r = 2.0f * kU * (fabsf(sx) + fabsf(sy) + fabsf(sz));
+  L.eP += r;
+  L.eT += r;
+  L.cell = nc;
+  L.frame = nc;
+}
+
+// A reflection mirrors both the exact and the FP32 continuation about the
+// face; the two mirror planes differ by the FP32 rounding of the face
+// offset, which is all a reflection adds to the error.
+__device__ __forceinline__ void reflect_error(const CellGeo*, uint32_t, Live<double>&) {}
+__device__ __forceinline__ void reflect_error(const CellGeo* geo, uint32_t axis, Live<float>& L) {
+  const float w = (float)(geo->nodes[2 * axis + 1] - geo->nodes[2 * axis]);
+  L.eP += 2.0f * kU * w;
+  L.eT += 2.0f * kU * w;
+}
+
 template <typename R>
-__device__ __forceinline__ bool boundary_event(const DevCells& cells, uint32_t bc_bits,
+__device__ __forceinline__ bool boundary_event(const DevCells& cells, const CellGeo* geo,
                                                Live<R>& L, Flush& fo) {
-  const int bc = face_bc(bc_bits, L.sc);
+  const int bc = face_bc(geo->bc_bits, L.sc);
   if (bc == kElement) {
     flush(fo, L);
-    L.cell = cells.next[(size_t)L.cell * 6 + L.sc];
+    enter_cell(cells, geo, cells.next[(size_t)L.cell * 6 + L.sc], L);
     L.desc = kBound;
     L.t_abs = 0.0;
     L.t_track = 0.0;
@@ -144,6 +225,7 @@ __device__ __forceinline__ bool boundary_event(const DevCells& cells, uint32_t b
   L.u = (axis == 0) ? -L.u : L.u;
   L.v = (axis == 1) ? -L.v : L.v;
   L.w = (axis == 2) ? -L.w : L.w;
+  reflect_error(geo, axis, L);
   L.desc = kBound;
   return false;
 }
@@ -151,9 +233,31 @@ __device__ __forceinline__ bool boundary_event(const DevCells& cells, uint32_t b
 // Draw states of a segment.
 enum : int { kDrawMu = 0, kDrawPhi, kDrawTest, kDrawGroup, kDrawDistance, kDrawDone };
 
+// What one FP64 segment tells the adjoint (filled when kInfo is set).
+struct SegInfo {
+  uint32_t c;            // cell of the segment
+  double sa, f, sig;     // sigma_a, Fleck factor, sigma_eff = (1 - f) sa + ss
+  double d, E_in, absorbed, kill_E;
+  bool flight;           // a flight was sampled (sigma_eff > 0)
+  bool scattered;        // ... and ended in a collision (d == d_s)
+  bool killed;
+  uint8_t reemit;        // re-emission test drawn this segment: 0 no, 1 kept, 2 re-sampled
+  double r_sa, r_f, r_sig;  // the opacities that test used
+};
+
 // ---- FP64: the reference's arithmetic, operation for operation ----------
-template <bool kTrack>
-__device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, Flush& fo) {
+template <bool kTrack, bool kInfo = false>
+__device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, Flush& fo,
+                                        SegInfo* si = nullptr) {
+  if constexpr (kInfo) {
+    si->reemit = 0;
+    si->flight = false;
+    si->killed = false;
+    si->absorbed = 0.0;
+    si->kill_E = 0.0;
+    si->d = 0.0;
+    si->c = L.cell;
+  }
   if (++L.segs > kMaxSegments) {
     flush(fo, L);
     L.desc = kStuck;
@@ -184,7 +288,14 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
       L.w = mu;
       st = kDrawTest;
     } else if (st == kDrawTest) {  // re-emission test (:97)
-      st = (r > (sg.y / total_sigma_s)) ? kDrawGroup : kDrawDistance;
+      const bool resample = r > (sg.y / total_sigma_s);
+      if constexpr (kInfo) {
+        si->reemit = resample ? 2 : 1;
+        si->r_sa = sg.x;
+        si->r_f = f;
+        si->r_sig = total_sigma_s;
+      }
+      st = resample ? kDrawGroup : kDrawDistance;
     } else if (st == kDrawGroup) {  // sample_emission_group walk (:98)
       const double2* sa = &cells.sigma[(size_t)L.cell * G];
       double cdf = r;
@@ -230,6 +341,17 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
   const double d = (m_bc < d_s) ? m_bc : d_s;    // std::min(d_s, .)
 
   const double absorbed = L.E * (1.0 - exp(-sigma_a * f * d));
+  if constexpr (kInfo) {
+    si->c = L.cell;
+    si->sa = sigma_a;
+    si->f = f;
+    si->sig = total_sigma_s;
+    si->d = d;
+    si->E_in = L.E;
+    si->absorbed = absorbed;
+    si->flight = total_sigma_s > 0.0;
+    si->scattered = (d == d_s);
+  }
   L.t_abs += absorbed;
   if constexpr (kTrack) L.t_track += absorbed / (sigma_a * f);
   L.E = L.E - absorbed;
@@ -243,6 +365,10 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
     flush(fo, L);
     fo.kill = L.E;
     L.desc = kKilled;
+    if constexpr (kInfo) {
+      si->killed = true;
+      si->kill_E = L.E;
+    }
     return true;
   }
   if (d == d_s) {  // effective scatter (:95-100); draws taken next segment
@@ -250,7 +376,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
     L.desc = kScatter;
     return false;
   }
-  if (d == d_b) return boundary_event(cells, geo->bc_bits, L, fo);
+  if (d == d_b) return boundary_event(cells, geo, L, fo);
   if (d == d_c) {  // census (:133-138)
     L.desc = kCensus;
     flush(fo, L);
@@ -260,6 +386,26 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
 }
 
 // ---- FP32 tracking ------------------------------------------------------
+// The segment of the FP64 form with FP32 geometry, plus the error bounds
+// that decide whether FP32 may take each decision.  The position error e
+// (FP32 minus exact, from the same start) is bounded by its components
+// perpendicular to the ray (eP) and along it (eT):
+//   * a move by d adds the chosen distance's error along the ray, and the
+//     FP32 direction's error (kDir per unit length) across it;
+//   * a face hit puts the photon on the face: the along-ray error becomes
+//     the face distance's evaluation error plus the perpendicular error seen
+//     through the face (eP / |u_axis|), while eP is unchanged -- parallel
+//     rays keep their separation, so errors do not compound per crossing;
+//   * a new direction (scatter) re-splits |e| <= eP + eT;
+//   * every FP32 update adds its rounding (kU per term).
+// Distance errors: to a face (eP + kU w + kDir |d|) / |u_axis| + eT + 4 kU |d|
+// (node offset, subtraction, reciprocal, product); to scatter
+// kE d_s + 2 kU / sigma (u rounded to FP32 moves log u by kU); to census eL.
+// A decision is ambiguous when the gap between the chosen candidate and any
+// other is within twice the sum of their bounds.  A segment whose deposit
+// could be off by more than fp32_tol (relative) is treated the same way, so
+// every FP32 deposit -- and with it every cell's tally -- is within fp32_tol
+// of FP64.
 template <bool kTrack>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, Flush& fo) {
   if (++L.segs > kMaxSegments) {
@@ -273,7 +419,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
   double2 sg = cells.sigma[(size_t)L.cell * G + L.group];
   double total_sigma_s = (1.0 - f) * sg.x + sg.y;
 
-  float d_s = 1.0e30f;
+  float d_s = 1.0e30f, e_s = 0.0f;
   int st = L.pending ? kDrawMu : kDrawDistance;
   if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
   double mu = 0.0;
@@ -286,13 +432,24 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
       // sin(theta) from the FP64 cosine: in FP32, mu can round to +-1 and
       // leave a direction with two exactly-zero components, whose face
       // distances are -inf.  In FP64 sqrt(1 - mu^2) >= 2^-25.5 is never 0.
-      const float phi = (float)(r * 2.0 * kPi);
+      // phi = 2 pi r, split as 4r = q + f with q = rint(4r) and |f| <= 1/2
+      // (exact in FP64): phi = q pi/2 + f pi/2, so the FP32 argument of
+      // sincospif is below pi/4 and rounds by at most 2^-25, and the quarter
+      // turns are exact swaps and sign changes
+      const double r4 = r * 4.0;
+      const double q = rint(r4);
+      float s0, c0;
+      sincospif(0.5f * (float)(r4 - q), &s0, &c0);
+      const int qi = (int)q & 3;
+      const float cp = (qi == 0) ? c0 : (qi == 1) ? -s0 : (qi == 2) ? -c0 : s0;
+      const float sp = (qi == 0) ? s0 : (qi == 1) ? c0 : (qi == 2) ? -s0 : -c0;
       const float s_t = (float)sqrt(1.0 - mu * mu);
-      float sp, cp;
-      sincosf(phi, &sp, &cp);
       L.u = s_t * cp;
       L.v = s_t * sp;
       L.w = (float)mu;
+      const float e = L.eP + L.eT;  // |e| seen from the new direction
+      L.eP = e;
+      L.eT = e;
       st = kDrawTest;
     } else if (st == kDrawTest) {
       st = (r > (sg.y / total_sigma_s)) ? kDrawGroup : kDrawDistance;
@@ -316,61 +473,111 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
     } else {
       float uf = (float)r;
       uf = (uf < 1.0f) ? uf : 0x1.fffffep-1f;  // (0,1) stays open after rounding
-      d_s = -__logf(uf) / (float)total_sigma_s;
+      const float sig = (float)total_sigma_s;
+      d_s = -logf(uf) / sig;
+      e_s = kE * d_s + 2.0f * kU / sig;
       st = kDrawDone;
     }
   }
   L.pending = false;
   const double saf = sg.x * f;
 
-  float d_b = 1.0e16f;
+  // faces in the cell's frame: low faces at 0, high faces at the width;
+  // the strict-< axis order of the reference picks the face, and the
+  // runner-up axis is kept for the face ambiguity test
+  float d_b = 1.0e16f, e_b = 0.0f, d_b2 = 1.0e16f, e_b2 = 0.0f;
   {
     const bool px = 0.0f < L.u, py = 0.0f < L.v, pz = 0.0f < L.w;
-    const float nx = (float)(px ? geo->nodes[1] : geo->nodes[0]);
-    const float ny = (float)(py ? geo->nodes[3] : geo->nodes[2]);
-    const float nz = (float)(pz ? geo->nodes[5] : geo->nodes[4]);
-    const float dx = (nx - L.x) / L.u;
-    if (dx < d_b) { d_b = dx; L.sc = px ? 1u : 0u; }
-    const float dy = (ny - L.y) / L.v;
-    if (dy < d_b) { d_b = dy; L.sc = py ? 3u : 2u; }
-    const float dz = (nz - L.z) / L.w;
-    if (dz < d_b) { d_b = dz; L.sc = pz ? 5u : 4u; }
+    const float wx = (float)(geo->nodes[1] - geo->nodes[0]);
+    const float wy = (float)(geo->nodes[3] - geo->nodes[2]);
+    const float wz = (float)(geo->nodes[5] - geo->nodes[4]);
+    const float ix = 1.0f / L.u, iy = 1.0f / L.v, iz = 1.0f / L.w;
+    const float dx = ((px ? wx : 0.0f) - L.x) * ix;
+    const float dy = ((py ? wy : 0.0f) - L.y) * iy;
+    const float dz = ((pz ? wz : 0.0f) - L.z) * iz;
+    const float ex = (L.eP + kU * wx + kDir * fabsf(dx)) * fabsf(ix) + L.eT + 4.0f * kU * fabsf(dx);
+    const float ey = (L.eP + kU * wy + kDir * fabsf(dy)) * fabsf(iy) + L.eT + 4.0f * kU * fabsf(dy);
+    const float ez = (L.eP + kU * wz + kDir * fabsf(dz)) * fabsf(iz) + L.eT + 4.0f * kU * fabsf(dz);
+    if (dx < d_b) { d_b2 = d_b; e_b2 = e_b; d_b = dx; e_b = ex; L.sc = px ? 1u : 0u; }
+    else if (dx < d_b2) { d_b2 = dx; e_b2 = ex; }
+    if (dy < d_b) { d_b2 = d_b; e_b2 = e_b; d_b = dy; e_b = ey; L.sc = py ? 3u : 2u; }
+    else if (dy < d_b2) { d_b2 = dy; e_b2 = ey; }
+    if (dz < d_b) { d_b2 = d_b; e_b2 = e_b; d_b = dz; e_b = ez; L.sc = pz ? 5u : 4u; }
+    else if (dz < d_b2) { d_b2 = dz; e_b2 = ez; }
   }
-  const float d_c = L.life;
+  const float d_c = L.life, e_c = L.eL;
   const float m_bc = (d_c < d_b) ? d_c : d_b;
   const float d = (m_bc < d_s) ? m_bc : d_s;
+  const bool is_s = (d == d_s);
+  const bool is_b = !is_s && (d == d_b);
+  const bool is_c = !is_s && !is_b;
+  const float e_d = is_s ? e_s : (is_b ? e_b : e_c);
+  bool amb = (!is_s && (d_s - d) <= 2.0f * (e_s + e_d)) ||
+             (!is_b && (d_b - d) <= 2.0f * (e_b + e_d)) ||
+             (!is_c && (d_c - d) <= 2.0f * (e_c + e_d)) ||
+             (is_b && d_b2 < 1.0e16f && (d_b2 - d_b) <= 2.0f * (e_b + e_b2));
 
-  // 1 - exp(-x) as -expm1(-x): no cancellation for the small optical depths
-  // of a single segment; the weight and tallies stay FP64.
-  const float frac = -expm1f(-(float)saf * d);
-  const double absorbed = L.E * (double)frac;
+  // implicit capture with the weight in FP64: the absorbed fraction from
+  // -expm1f below ln 2, the surviving fraction from expf above, so neither
+  // small nor large optical depths lose digits to cancellation
+  const float x = (float)saf * d;
+  double absorbed, E_new;
+  float eE = L.eE + (float)saf * e_d;
+  if (x < 0.69314718f) {
+    absorbed = L.E * (double)(-expm1f(-x));
+    E_new = L.E - absorbed;
+    eE += kE * fabsf(x);
+  } else {
+    E_new = L.E * (double)expf(-x);
+    absorbed = L.E - E_new;
+    eE += kE * (x + 1.0f);
+  }
+  const double ratio = E_new / L.E0;
+  amb = amb || (fabs(ratio - kCutoff) <= 2.0 * (double)eE * ratio);
+  // accuracy cap: the deposit's relative error is bounded by the distance's
+  // relative bound plus the weight's; past fp32_tol the history goes to FP64
+  // (also catches d = 0 and faces met at grazing incidence)
+  // (kE / 2 covers expm1f / expf and the FP32 products of the exponent)
+  amb = amb || !(e_d + (eE + 0.5f * kE) * fabsf(d) <= cells.fp32_tol * fabsf(d));
+  if (amb) {  // stop before this segment; FP64 replays from here on
+    flush(fo, L);
+    L.first_seg = L.segs;
+    L.E_handoff = L.E;
+    L.desc = kReplay;
+    return true;
+  }
   L.t_abs += absorbed;
   if constexpr (kTrack) L.t_track += absorbed / saf;
-  L.E = L.E - absorbed;
+  L.E = E_new;
+  L.eE = eE;
 
   L.x += L.u * d;
   L.y += L.v * d;
   L.z += L.w * d;
   L.life -= d;
+  {
+    const float ad = fabsf(d);
+    const float rnd = kU * (fabsf(L.x) + fabsf(L.y) + fabsf(L.z) + 2.0f * ad);
+    L.eT = (is_b ? e_d - L.eT : L.eT + e_d) + rnd;
+    L.eP += kDir * ad + rnd;
+    L.eL += e_d + kU * fabsf(L.life);
+  }
 
-  if (L.E / L.E0 < kCutoff) {
+  if (ratio < kCutoff) {
     flush(fo, L);
     fo.kill = L.E;
     L.desc = kKilled;
     return true;
   }
-  if (d == d_s) {
+  if (is_s) {
     L.pending = true;
     L.desc = kScatter;
     return false;
   }
-  if (d == d_b) return boundary_event(cells, geo->bc_bits, L, fo);
-  if (d == d_c) {
-    L.desc = kCensus;
-    flush(fo, L);
-    return true;
-  }
-  return false;
+  if (is_b) return boundary_event(cells, geo, L, fo);
+  L.desc = kCensus;
+  flush(fo, L);
+  return true;
 }
 
 __device__ __forceinline__ void add_counters(DevCounters* ctrs, unsigned long long segs,
@@ -400,26 +607,64 @@ __device__ __forceinline__ void bind_lds(Tally& tally, uint32_t n_cells) {
 // (src/history_based_transport.h:263-275) over SoA state.  The segment loop
 // runs while any lane of the wave is active, so tally pushes stay
 // wave-uniform.
+// Wave-aggregated append of the lanes with `want` set to the replay list:
+// one atomic per wave on the list length.  Called at wave-uniform points.
+__device__ __forceinline__ void replay_append(bool want, uint64_t idx, uint32_t first_seg,
+                                              double E, const ReplayList& rl,
+                                              DevCounters* ctrs) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(&ctrs->replays, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (want) {
+    const uint64_t slot = base + lane_rank(m);
+    rl.idx[slot] = idx;
+    rl.first_seg[slot] = first_seg;
+    rl.E[slot] = E;
+  }
+}
+
+// End of a history inside a tracking kernel: store the photon, or -- FP32
+// form, ambiguous decision -- leave its initial state in place for the
+// replay.  Returns the segments this kernel applied.
+template <typename R>
+__device__ __forceinline__ uint32_t end_history(const DevCells& cells, const PhotonSoA& W,
+                                                uint64_t i, const Live<R>& L, bool& replay) {
+  if constexpr (sizeof(R) == 4) {
+    if (L.desc == kReplay) {
+      replay = true;
+      return L.segs - 1;
+    }
+  }
+  store_live(cells, W, i, L);
+  return L.segs;
+}
+
 template <class Tally, typename R>
 BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n,
-                                         uint64_t ctr_hi, Tally tally, DevCounters* ctrs) {
+                                         uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
+                                         ReplayList rl) {
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool active = i < n;
   unsigned long long flushes = 0, segs = 0, stuck = 0;
   Live<R> L;
-  if (active) load_live(W, i, ctr_hi, L);
+  if (active) load_live(cells, W, i, ctr_hi, L);
   while (__ballot(active) != 0) {
     Flush fo;
     fo.has = false;
     fo.kill = 0.0;
+    bool replay = false;
     if (active && segment<Tally::kTrack>(cells, L, fo)) {
-      segs = L.segs;
       stuck = (L.desc == kStuck);
-      store_live(W, i, L);
+      segs = end_history(cells, W, i, L, replay);
       active = false;
     }
+    if constexpr (sizeof(R) == 4) replay_append(replay, i, L.first_seg, L.E_handoff, rl, ctrs);
     flushes += fo.has;
     tally.push(fo);
   }
@@ -435,7 +680,7 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
 template <class Tally, typename R>
 BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_t n,
                                             uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
-                                            PersistentQueue* q, int refill) {
+                                            PersistentQueue* q, int refill, ReplayList rl) {
   constexpr uint64_t kChunk = 256;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
@@ -468,7 +713,7 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
         const bool take = !active && rank < avail;
         if (take) {
           idx = chunk_next + rank;
-          load_live(W, idx, ctr_hi, L);
+          load_live(cells, W, idx, ctr_hi, L);
           active = true;
         }
         const uint64_t taken = (avail < (uint64_t)n_idle) ? avail : (uint64_t)n_idle;
@@ -481,16 +726,188 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
     Flush fo;
     fo.has = false;
     fo.kill = 0.0;
+    bool replay = false;
     if (active && segment<Tally::kTrack>(cells, L, fo)) {
-      segs += L.segs;
       stuck += (L.desc == kStuck);
-      store_live(W, idx, L);
+      segs += end_history(cells, W, idx, L, replay);
       active = false;
     }
+    if constexpr (sizeof(R) == 4) replay_append(replay, idx, L.first_seg, L.E_handoff, rl, ctrs);
     flushes += fo.has;
     tally.push(fo);
   }
   tally.finish(cells.n_cells);
+  add_counters(ctrs, segs, flushes, stuck);
+}
+
+// FP64 replay of the histories the FP32 form stopped: each starts again
+// from its initial state, and the deposits and flushes of the segments
+// before first_seg -- already tallied in FP32 -- are muted; the
+// accumulators restart at first_seg, whose partial cell visit the FP32 pass
+// flushed.  From first_seg on, deposits and the final weight are scaled by
+// s = (FP32 weight) / (FP64 weight) at that point, a factor within rounding
+// of 1: FP32 deposits + s * (FP64 deposits + final weight) then equal the
+// initial weight, so conservation holds to round-off, while every event
+// (the cutoff test included) still uses the unscaled FP64 weight.  The list
+// length is read on the device, so no host round trip separates the two
+// launches.  Loop bounds are block-uniform.
+template <class Tally>
+BH_TRACK_KERNEL void k_transport_replay(DevCells cells, PhotonSoA W, ReplayList rl,
+                                        uint64_t ctr_hi, Tally tally, DevCounters* ctrs) {
+  bind_lds(tally, cells.n_cells);
+  tally.init(cells.n_cells);
+  const uint64_t n = ctrs->replays;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long flushes = 0, segs = 0, stuck = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint64_t j = base + threadIdx.x;
+    bool active = j < n;
+    uint64_t i = 0;
+    Live<double> L;
+    if (active) {
+      i = rl.idx[j];
+      load_live(cells, W, i, ctr_hi, L);
+      L.first_seg = rl.first_seg[j];
+      L.E_handoff = rl.E[j];
+    }
+    while (__ballot(active) != 0) {
+      Flush fo;
+      fo.has = false;
+      fo.kill = 0.0;
+      if (active) {
+        if (L.segs + 1 == L.first_seg) {
+          L.t_abs = 0.0;
+          L.t_track = 0.0;
+          L.E_handoff /= L.E;  // from here on: the weight scale s
+        }
+        const bool end = segment<Tally::kTrack>(cells, L, fo);
+        if (L.segs < L.first_seg) {
+          fo.has = false;
+          fo.kill = 0.0;
+        } else {
+          fo.cont *= L.E_handoff;
+          fo.track *= L.E_handoff;
+          fo.kill *= L.E_handoff;
+        }
+        if (end) {
+          stuck += (L.desc == kStuck);
+          segs += L.segs + 1 - L.first_seg;
+          L.E *= L.E_handoff;
+          store_live(cells, W, i, L);
+          active = false;
+        }
+      }
+      flushes += fo.has;
+      tally.push(fo);
+    }
+  }
+  tally.finish(cells.n_cells);
+  add_counters(ctrs, segs, flushes, stuck);
+}
+
+// Path-replay adjoint of the absorbed-energy tally.  For a loss L with
+// g[c] = dL/d abs_E[c], the likelihood-ratio (differential operator)
+// derivative of the expected tallies, source photons held fixed, is
+//   dL/dtheta = sum over histories of
+//     sum over decisions e:  d log p_e / dtheta * R_e
+//   + sum over segments j:   dtau_j / dtheta * (g_c E_out_j - R_after_j),
+// where R_e is the g-weighted score of the history from decision e on (the
+// score of the segment a flight length decides included), R_after_j the
+// g-weighted score after segment j (a cutoff deposit is the weight left
+// after its segment, so it counts as "after"), and tau_j = sa f d the
+// segment's optical depth.  The decisions that depend on a cell's sigma_a
+// and Fleck factor f (sigma_eff = (1 - f) sa + ss):
+//   flight:   collision at d: d log p = dsigma_eff (1/sigma_eff - d);
+//             no collision within d: d log p = -dsigma_eff d;
+//   re-emission test: group kept with probability ss / sigma_eff,
+//             re-sampled with (1 - f) sa / sigma_eff;
+// dsigma_eff/dsa = 1 - f, dsigma_eff/df = -sa, dtau/dsa = f d,
+// dtau/df = sa d.  (Group-independent sigma_a: the group walk's
+// probabilities then do not depend on the parameters.  The cutoff test is a
+// discontinuity and contributes nothing.)
+//
+// No trajectory is stored: pass 1 runs each history from its initial state
+// (its Random123 counter) to get its total R; pass 2 replays it with the
+// same draws and, knowing the score accumulated so far, forms every R_e as
+// total minus accumulated.  Per-cell gradients are summed over a cell visit
+// and flushed through the tally sink: sink "abs" = dL/dsigma_a, sink
+// "track" = dL/dfleck.
+template <class Sink>
+BH_TRACK_KERNEL void k_adjoint(DevCells cells, PhotonSoA W, uint64_t n, uint64_t ctr_hi,
+                               const double* g, Sink sink, DevCounters* ctrs) {
+  bind_lds(sink, cells.n_cells);
+  sink.init(cells.n_cells);
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long segs = 0, flushes = 0, stuck = 0;
+  Live<double> L;
+  SegInfo si;
+  // pass 1: total g-weighted score of the history
+  bool active = i < n;
+  double R = 0.0;
+  if (active) load_live(cells, W, i, ctr_hi, L);
+  while (active) {
+    Flush fo;
+    fo.has = false;
+    const bool end = segment<false, true>(cells, L, fo, &si);
+    R += g[si.c] * (si.absorbed + si.kill_E);
+    if (end) {
+      stuck = (L.desc == kStuck);
+      active = false;
+    }
+  }
+  // pass 2: replay and accumulate the gradient
+  active = i < n;
+  double A = 0.0, gs = 0.0, gf = 0.0;
+  if (active) load_live(cells, W, i, ctr_hi, L);
+  while (__ballot(active) != 0) {
+    Flush out;
+    out.has = false;
+    out.kill = 0.0;
+    if (active) {
+      Flush fo;
+      fo.has = false;
+      const bool end = segment<false, true>(cells, L, fo, &si);
+      const double gc = g[si.c];
+      if (si.reemit) {  // decided before this segment's flight
+        const double Rr = R - A;
+        const double a1 = 1.0 - si.r_f;
+        if (si.reemit == 2) {  // re-sampled: p = (1 - f) sa / sigma_eff
+          gs += (1.0 / si.r_sa - a1 / si.r_sig) * Rr;
+          gf += (-1.0 / a1 + si.r_sa / si.r_sig) * Rr;
+        } else {  // kept: p = ss / sigma_eff
+          gs += (-a1 / si.r_sig) * Rr;
+          gf += (si.r_sa / si.r_sig) * Rr;
+        }
+      }
+      const double s_own = gc * si.absorbed;
+      const double Rj = R - A;
+      const double Rafter = Rj - s_own;
+      if (si.flight) {
+        const double w = si.scattered ? (1.0 / si.sig - si.d) : -si.d;
+        gs += (1.0 - si.f) * w * Rj;
+        gf += -si.sa * w * Rj;
+      }
+      const double att = gc * (si.E_in - si.absorbed) - Rafter;
+      gs += si.f * si.d * att;
+      gf += si.sa * si.d * att;
+      A += s_own + gc * si.kill_E;
+      if (end || L.cell != si.c) {
+        out.has = true;
+        out.c = si.c;
+        out.cont = gs;
+        out.track = gf;
+        gs = 0.0;
+        gf = 0.0;
+      }
+      if (end) {
+        segs = L.segs;
+        active = false;
+      }
+    }
+    flushes += out.has;
+    sink.push(out);
+  }
+  sink.finish(cells.n_cells);
   add_counters(ctrs, segs, flushes, stuck);
 }
 

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BH_ABI_VERSION 1
+#define BH_ABI_VERSION 2
 
 /* return codes */
 #define BH_OK 0
@@ -46,7 +46,10 @@ enum { BH_EXIT = 0, BH_PASS = 1, BH_CENSUS = 2, BH_SCATTER = 3, BH_KILLED = 4, B
 /* arithmetic of the tracking loop */
 enum {
   BH_PRECISION_FP64 = 0,      /* everything FP64: replays the reference     */
-  BH_PRECISION_FP32_TRACK = 1 /* FP32 position/direction/path, FP64 energy  */
+  BH_PRECISION_FP32_TRACK = 1 /* FP32 position/direction/path, FP64 energy and
+                                 tallies; a history whose next event FP32
+                                 cannot decide is finished in FP64, so every
+                                 event matches the FP64 form               */
 };
 /* loop organisation */
 enum {
@@ -65,6 +68,11 @@ typedef struct bh_options {
   int32_t refill_threshold; /* persistent: idle lanes of a wave that trigger
                                a refill (1..64), 0 = default (16)          */
   uint64_t reserve_photons; /* initial photon capacity hint, 0 = none      */
+  double fp32_tol;          /* FP32 tracking: largest relative error bound a
+                               single deposit (and the photon weight) may
+                               carry; a history that would exceed it is
+                               finished in FP64.  Bounds every cell's tally
+                               error against FP64.  0 = default (1e-6)     */
 } bh_options;
 
 /* Read-only cell data for one step (src/cell.h:319-336). */
@@ -90,9 +98,12 @@ typedef struct bh_step_stats {
   uint64_t n_killed;
   uint64_t n_segments;    /* iterations of the tracking loop              */
   uint64_t n_flushes;     /* tally flushes (cell exits and terminations)  */
-  double transport_ms;    /* device time of the tracking kernel           */
+  double transport_ms;    /* device time of the tracking kernel(s)        */
   double census_ms;       /* device time of census compaction             */
   double source_ms;       /* device time of the last bh_source call       */
+  uint64_t n_replayed;    /* FP32 tracking: histories whose next decision
+                             the FP32 error bound could not separate,
+                             finished in FP64 from that segment on        */
 } bh_step_stats;
 
 /* A photon in the reference's 120-byte AOS layout (src/photon.h:172-181). */
@@ -187,6 +198,20 @@ int bh_census_import(bh_handle *h, const bh_photon *in, uint64_t n);
  * added to, not overwritten).  Does not touch the resident census. */
 int bh_transport_photons(bh_handle *h, bh_photon *photons, uint64_t n,
                          double *abs_E, double *track_E, bh_step_stats *stats);
+
+/* Path-replay adjoint of the absorbed-energy tally (no counterpart in the
+ * reference).  For photons in their state BEFORE transport (the input of
+ * bh_transport_photons), the cells of the last bh_set_mesh and
+ * abs_adjoint[c] = dL/d abs_E[c] of a loss L on the absorbed-energy tally,
+ * accumulates (adds) into d_sigma_a[c] and d_fleck[c] the derivative of the
+ * expected L with respect to the cell's absorption opacity (all groups
+ * together) and Fleck factor, source photons held fixed: the
+ * differential-operator (likelihood-ratio) estimator, computed by replaying
+ * each history from its Random123 counter, with no trajectory stored.  FP64.
+ * Requires sigma_a equal in every group of a cell.  n_cells doubles each. */
+int bh_adjoint_photons(bh_handle *h, const bh_photon *photons, uint64_t n,
+                       const double *abs_adjoint, double *d_sigma_a, double *d_fleck,
+                       bh_step_stats *stats);
 
 /* Counter-based RNG as the device computes it: n draws of RNG(seed, stream)
  * from counter ctr0 (src/RNG.h:262-342).  Runs on the device. */

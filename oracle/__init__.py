@@ -159,6 +159,8 @@ def lib() -> C.CDLL:
         L.orc_make_initial_census.restype = U64
         L.orc_transport.argtypes = [P, P, U64, P, P, I]
         L.orc_transport.restype = U64
+        L.orc_dos_transport.argtypes = [P, P, U64, P, P, P, P, I]
+        L.orc_dos_transport.restype = U64
         L.orc_post_process.argtypes = [D, P, U64, P, U64, P]
         L.orc_post_process.restype = U64
         L.orc_sim_create.argtypes = [P]
@@ -322,6 +324,21 @@ def transport(cells: Cells, photons: np.ndarray, n_threads: int = 1):
     segs = lib().orc_transport(cells.ref, _ptr(photons), len(photons), _ptr(abs_E),
                                _ptr(track_E), n_threads)
     return abs_E, track_E, int(segs)
+
+
+def dos_transport(cells: Cells, photons: np.ndarray, v_sigma_a, v_fleck, n_threads: int = 1):
+    """Transport in place with the forward-mode differential-operator tally
+    along (v_sigma_a, v_fleck); returns (abs_E, dabs_E, n_segments), where
+    dabs_E[c] = d abs_E[c] / dt for sigma_a + t v_sigma_a, fleck + t v_fleck."""
+    assert photons.dtype == PHOTON_DTYPE and photons.flags.c_contiguous
+    va = np.ascontiguousarray(v_sigma_a, dtype=np.float64)
+    vf = np.ascontiguousarray(v_fleck, dtype=np.float64)
+    assert va.shape == (cells.n_cells,) and vf.shape == (cells.n_cells,)
+    abs_E = np.zeros(cells.n_cells)
+    dabs_E = np.zeros(cells.n_cells)
+    segs = lib().orc_dos_transport(cells.ref, _ptr(photons), len(photons), _ptr(va), _ptr(vf),
+                                   _ptr(abs_E), _ptr(dabs_E), n_threads)
+    return abs_E, dabs_E, int(segs)
 
 
 def post_process(next_dt: float, photons: np.ndarray):

@@ -506,6 +506,153 @@ uint64_t orc_transport(const orc_cells *cells, orc_photon *photons,
   return segs;
 }
 
+/* ======================================================================== */
+/* Differential-operator (likelihood-ratio) tally, forward mode             */
+/* ======================================================================== */
+/* Derivative of the expected absorbed-energy tally along a direction of the
+ * per-cell parameters: sigma_a(c) + t v_sa(c), fleck(c) + t v_f(c), t -> 0,
+ * with the source photons held fixed.  It is the classic differential
+ * operator estimator of Monte Carlo perturbation theory: every score s of a
+ * history contributes
+ *     s * S + ds/dt|path fixed,
+ * where S sums d log p / dt over the random decisions made before (and
+ * including) the one the score depends on, and ds/dt|path fixed is the
+ * derivative through the implicit-capture weights.  The decisions that
+ * depend on the parameters in a cell with sigma_eff = (1 - f) sa + ss:
+ *   flight: collision at d, density sigma_eff exp(-sigma_eff d):
+ *           d log p = dsigma_eff (1/sigma_eff - d); no collision within d
+ *           (face or census first): d log p = -dsigma_eff d;
+ *   re-emission test after a collision: keep the group with probability
+ *           ss / sigma_eff, re-sample with (1 - f) sa / sigma_eff.
+ * dsigma_eff = (1 - f) v_sa - sa v_f; the optical depth of a segment
+ * tau = sa f d has dtau = (f v_sa + sa v_f) d.  Group-independent sigma_a
+ * only: then the group walk draws each group with a probability that does
+ * not depend on the parameters.  The cutoff test is a discontinuity of the
+ * estimator and contributes nothing here.
+ *
+ * The photons are transported in place with exactly the events of
+ * orc_transport (the same statements in the same order, plus the derivative
+ * bookkeeping).  abs_E and dabs_E are accumulated into. */
+static uint64_t orc_dos_photon(const orc_cells *cells, orc_photon *p,
+                               const double *v_sa, const double *v_f,
+                               double *abs_E, double *dabs_E) {
+  const uint32_t G = cells->n_groups;
+  uint64_t *rng = p->rng;
+  uint32_t sc = 0;
+  uint32_t c = p->cell;
+  uint64_t segs = 0;
+  double S = 0.0;   /* score function: sum of d log p / dt so far */
+  double dlE = 0.0; /* d log(weight) / dt through implicit capture  */
+  int active = 1;
+  while (active) {
+    ++segs;
+    const size_t cg = (size_t)c * G + p->group;
+    const double sigma_s = cells->sigma_s[cg];
+    const double sigma_a = cells->sigma_a[cg];
+    const double f = cells->fleck[c];
+    const double total_sigma_s = (1.0 - f) * sigma_a + sigma_s;
+    const double dsig = (1.0 - f) * v_sa[c] - sigma_a * v_f[c];
+
+    const double d_s =
+        (total_sigma_s > 0.0) ? -log(orc_rng_next(rng)) / total_sigma_s
+                              : 1.0e100;
+    const double d_b =
+        orc_dist_boundary(&cells->nodes[6 * (size_t)c], p->pos, p->angle, &sc);
+    const double d_c = p->life_dx;
+    const double m_bc = (d_c < d_b) ? d_c : d_b;
+    const double d = (m_bc < d_s) ? m_bc : d_s;
+
+    if (total_sigma_s > 0.0)
+      S += dsig * ((d == d_s) ? (1.0 / total_sigma_s - d) : -d);
+    const double dtau = (f * v_sa[c] + sigma_a * v_f[c]) * d;
+
+    const double absorbed = p->E * (1.0 - exp(-sigma_a * f * d));
+    p->E = p->E - absorbed;
+    abs_E[c] += absorbed;
+    dabs_E[c] += absorbed * (S + dlE) + p->E * dtau;
+    dlE -= dtau;
+
+    p->pos[0] += p->angle[0] * d;
+    p->pos[1] += p->angle[1] * d;
+    p->pos[2] += p->angle[2] * d;
+    p->life_dx -= d;
+
+    if (p->E / p->E0 < ORC_CUTOFF) {
+      abs_E[c] += p->E;
+      dabs_E[c] += p->E * (S + dlE);
+      active = 0;
+      p->descriptor[0] = ORC_KILLED;
+    } else if (d == d_s) {
+      orc_uniform_angle(rng, p->angle);
+      if (orc_rng_next(rng) > (sigma_s / ((1.0 - f) * sigma_a + sigma_s))) {
+        /* (1 - f) sa = sigma_eff - ss, whose derivative is dsig */
+        S += dsig / ((1.0 - f) * sigma_a) - dsig / total_sigma_s;
+        p->group = orc_emission_group(&cells->sigma_a[(size_t)c * G], G, rng);
+      } else {
+        S -= dsig / total_sigma_s;
+      }
+      p->descriptor[0] = ORC_SCATTER;
+    } else if (d == d_b) {
+      const int32_t bc = cells->bc[6 * (size_t)c + sc];
+      if (bc == ORC_ELEMENT) {
+        c = cells->next[6 * (size_t)c + sc];
+        p->cell = c;
+        p->descriptor[0] = ORC_BOUND;
+      } else if (bc == ORC_PROCESSOR) {
+        active = 0;
+        p->cell = cells->next[6 * (size_t)c + sc];
+        p->descriptor[0] = ORC_PASS;
+      } else if (bc == ORC_VACUUM || bc == ORC_SOURCE) {
+        active = 0;
+        p->descriptor[0] = ORC_EXIT;
+      } else {
+        p->angle[sc / 2] = -p->angle[sc / 2];
+        p->descriptor[0] = ORC_BOUND;
+      }
+    } else if (d == d_c) {
+      active = 0;
+      p->descriptor[0] = ORC_CENSUS;
+    }
+  }
+  return segs;
+}
+
+uint64_t orc_dos_transport(const orc_cells *cells, orc_photon *photons,
+                           uint64_t n, const double *v_sigma_a,
+                           const double *v_fleck, double *abs_E,
+                           double *dabs_E, int n_threads) {
+  const uint32_t nc = cells->n_cells;
+  uint64_t segs = 0;
+  if (n_threads <= 1) {
+    for (uint64_t i = 0; i < n; ++i)
+      segs += orc_dos_photon(cells, &photons[i], v_sigma_a, v_fleck, abs_E,
+                             dabs_E);
+    return segs;
+  }
+  double *ta = (double *)calloc((size_t)n_threads * nc, sizeof(double));
+  double *td = (double *)calloc((size_t)n_threads * nc, sizeof(double));
+#pragma omp parallel num_threads(n_threads) reduction(+ : segs)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num();
+#else
+    const int t = 0;
+#endif
+#pragma omp for schedule(guided)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+      segs += orc_dos_photon(cells, &photons[i], v_sigma_a, v_fleck,
+                             ta + (size_t)t * nc, td + (size_t)t * nc);
+  }
+  for (uint32_t c = 0; c < nc; ++c)
+    for (int t = 0; t < n_threads; ++t) {
+      abs_E[c] += ta[(size_t)t * nc + c];
+      dabs_E[c] += td[(size_t)t * nc + c];
+    }
+  free(ta);
+  free(td);
+  return segs;
+}
+
 /* post_process_photons (src/post_process_functions.h:32-59): census keeps
  * transport order and gets life_dx = c * next_dt. */
 uint64_t orc_post_process(double next_dt, orc_photon *photons, uint64_t n,

@@ -120,6 +120,16 @@ uint64_t orc_transport(const orc_cells *cells, orc_photon *photons,
                        uint64_t n, double *abs_E, double *track_E,
                        int n_threads);
 
+/* Differential-operator (likelihood-ratio) derivative of the absorbed-energy
+ * tally along the direction (v_sigma_a, v_fleck) of the per-cell parameters,
+ * source photons held fixed; forward mode, same events as orc_transport.
+ * dabs_E[c] and abs_E[c] are accumulated into.  Group-independent sigma_a
+ * only.  Returns segments. */
+uint64_t orc_dos_transport(const orc_cells *cells, orc_photon *photons,
+                           uint64_t n, const double *v_sigma_a,
+                           const double *v_fleck, double *abs_E,
+                           double *dabs_E, int n_threads);
+
 /* post_process_photons (src/post_process_functions.h:32-59). */
 typedef struct orc_census_stats {
   double census_E;
