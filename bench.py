@@ -61,19 +61,19 @@ def _algorithmic_bytes(form, hist, segs, flushes):
             + b["flush"] * flushes)
 
 
-def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False):
+def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=DECK,
+              n_photons=N_PHOTONS, n_groups=1, fp32_tol=0.0):
     """Warm up on one driver, then time `steps` steps on a fresh one.  Photon
     buffers are reserved up front (device memory is set-up, not step work):
     the census grows to ~3.7x the photons per step over five steps."""
-    reserve = N_PHOTONS * (1 + min(steps, 8))
+    reserve = n_photons * (1 + min(steps, 8))
+    kw = dict(precision=precision, algorithm=algorithm, n_groups=n_groups, fp32_tol=fp32_tol)
     if warmup:
-        with capi.Driver(DECK, precision=precision, algorithm=algorithm,
-                         t_stop=DT * warmup) as w:
+        with capi.Driver(deck, t_stop=DT * warmup, **kw) as w:
             w.run()
     out = dict(hist=0, segs=0, flushes=0, dev_s=0.0, step_s=0.0, k_ms=[], launches=[],
-               step1=None)
-    with capi.Driver(DECK, precision=precision, algorithm=algorithm, t_stop=DT * steps,
-                     reserve_photons=reserve) as d:
+               step1=None, replayed=0)
+    with capi.Driver(deck, t_stop=DT * steps, reserve_photons=reserve, **kw) as d:
         eng = capi.lib()
         h = eng.bh_driver_engine(d._d)
         for k in range(steps):
@@ -89,6 +89,7 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False):
             if k == 0 and keep_step1:
                 out["step1"] = d.tallies()[0]
             out["hist"] += r["transported"]
+            out["replayed"] += r["n_replayed"]
             out["segs"] += r["n_segments"]
             out["flushes"] += r["n_flushes"]
             out["dev_s"] += t1 - t0
@@ -119,6 +120,20 @@ def _traffic_from_profile():
         return None
 
 
+def _tally_error(ref, got):
+    """Per-cell error of a tally against the FP64 form: max and 99th
+    percentile of |got - ref| / ref over cells with ref > 0, and the
+    normwise max |got - ref| / max ref."""
+    import numpy as np
+    m = ref > 0
+    if not m.any():
+        return {"max_rel": 0.0, "p99_rel": 0.0, "normwise": 0.0}
+    rel = np.abs(got[m] - ref[m]) / ref[m]
+    return {"max_rel": float(rel.max()), "p99_rel": float(np.percentile(rel, 99)),
+            "normwise": float(np.abs(got - ref).max() / ref.max()),
+            "zero_cells_differ": int(np.count_nonzero(got[~m]))}
+
+
 def _cpu_baseline(steps):
     """The oracle's OpenMP step (schedule(guided), per-thread tallies): the
     same source + transport + census phases over the same workload, on this
@@ -145,6 +160,116 @@ def _cpu_baseline(steps):
                       f"oracle C restatement, OpenMP guided, {threads} threads"}
 
 
+HOHLRAUM = os.path.join(ROOT, "branson_amd", "decks", "hohlraum_64.xml")
+
+
+def _common(metric, value, steps, warmup, ms, dtype, config, **extra):
+    line = {"metric": metric, "value": value, "unit": "histories/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": config}
+    line.update(extra)
+    return line
+
+
+def _bench_hohlraum(a, capi):
+    """30-group 64^3 hohlraum, 5e7 photons/step: plain FP64 against FP32
+    tracking (persistent waves) at the stated per-cell tally error."""
+    n_ph = 50_000_000
+    kw = dict(deck=HOHLRAUM, n_photons=n_ph, n_groups=30, keep_step1=True)
+    plain = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT, **kw)
+    ch = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP32_TRACK, capi.ALGO_PERSISTENT,
+                   fp32_tol=a.fp32_tol, **kw)
+    hist = plain["hist"] / plain["dev_s"]
+    chosen = {"form": "fp32 tracking + fp64 energies and tallies, persistent waves",
+              "fp32_tol": a.fp32_tol or 1e-6,
+              "histories_per_s": ch["hist"] / ch["dev_s"],
+              "segments_per_s": ch["segs"] / ch["dev_s"],
+              "ms_per_step": 1e3 * ch["dev_s"] / a.steps,
+              "speedup_vs_plain_fp64": (ch["hist"] / ch["dev_s"]) / hist,
+              "step1_tally_error_vs_fp64": _tally_error(plain["step1"], ch["step1"]),
+              "replayed_histories": ch["replayed"]}
+    return _common(METRIC, hist, a.steps, a.warmup, 1e3 * plain["dev_s"] / a.steps, "f64",
+                   {"workload": "hohlraum_64: 64^3 cells of 0.01 cm, 4-cell wall sigma_a=1000 "
+                                "rho=19, fill sigma_a=0.1 sigma_s=0.01 rho=0.001, 0.3 keV "
+                                "source on z=0, 30 groups, 5e7 photons/step",
+                    "form": "plain fp64, persistent waves",
+                    "histories": plain["hist"], "segments": plain["segs"]},
+                   segments_per_s=plain["segs"] / plain["dev_s"], chosen=chosen)
+
+
+def _bench_batched(a, capi):
+    """256 independent slab/cube problems in one engine, 3 steps."""
+    import numpy as np
+    from branson_amd import problems
+    from branson_amd.batched import BatchedRun
+
+    steps = min(a.steps, 3)
+    res = {}
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_FP32_TRACK):
+        b = problems.batched_instances()
+        with capi.Engine(0, algorithm=capi.ALGO_PERSISTENT, precision=prec,
+                         fp32_tol=a.fp32_tol,
+                         reserve_photons=int(b.n_photons.sum()) * 3) as e:
+            run = BatchedRun(b, e)
+            reps = [run.step() for _ in range(steps)]
+        res[prec] = reps
+    def rate(reps):
+        return sum(r["n_transported"] for r in reps) / sum(r["device_s"] for r in reps)
+    p64, p32 = res[capi.PRECISION_FP64], res[capi.PRECISION_FP32_TRACK]
+    chosen = {"form": "fp32 tracking + fp64 energies and tallies, persistent waves",
+              "fp32_tol": a.fp32_tol or 1e-6, "histories_per_s": rate(p32),
+              "speedup_vs_plain_fp64": rate(p32) / rate(p64),
+              "step1_tally_error_vs_fp64": _tally_error(p64[0]["abs_E"], p32[0]["abs_E"]),
+              "replayed_histories": sum(r["n_replayed"] for r in p32)}
+    return _common(METRIC, rate(p64), steps, 0,
+                   1e3 * sum(r["device_s"] for r in p64) / steps, "f64",
+                   {"workload": "batched: 256 slab/cube instances, cells log-uniform in "
+                                "[50, 32768], photons log-uniform in [1e4, 1e6], sigma_a "
+                                "lognormal (median 10, sigma_log 1), sigma_s U[0,2], T0 "
+                                "U[0.01,1] keV, dt 0.01 sh, seeds 0..255, one launch per "
+                                "phase for all instances",
+                    "form": "plain fp64, persistent waves",
+                    "histories": sum(r["n_transported"] for r in p64),
+                    "segments": sum(r["n_segments"] for r in p64)},
+                   chosen=chosen)
+
+
+def _bench_adjoint(a, capi):
+    """Forward tally and path-replay adjoint on the 32^3 lognormal field,
+    1e6 photons, one step."""
+    import numpy as np
+    from branson_amd import problems
+    from branson_amd.adjoint import FixedSource
+
+    prob = problems.adjoint_field_problem()
+    with capi.Engine(0, algorithm=capi.ALGO_PERSISTENT) as e:
+        src = FixedSource.sample(e, prob, 10**6, seed=2718)
+        target = src.absorbed(prob.sigma_a * 1.05, prob.fleck)
+        fwd_ms, adj_ms = [], []
+        for _ in range(max(a.steps, 1)):
+            src._set(prob.sigma_a, prob.fleck)
+            ph = src.photons.copy()
+            ab, _, st = e.transport_photons(ph)
+            fwd_ms.append(st["transport_ms"])
+            g = 2.0 * (ab - target)
+            src._set(prob.sigma_a, prob.fleck)
+            da, df, sa = e.adjoint_photons(src.photons, g)
+            adj_ms.append(sa["transport_ms"])
+    n = len(src.photons)
+    return _common("adjoint histories/s (two passes per history), 32^3 lognormal field",
+                   n / (np.mean(adj_ms) * 1e-3), max(a.steps, 1), 0, float(np.mean(adj_ms)),
+                   "f64",
+                   {"workload": "adjoint: 32^3 cells of 0.05 cm, sigma_a lognormal field "
+                                "(median 10, sigma_log 0.7, correlation 4 cells), sigma_s 0.5, "
+                                "T 0.5 keV, 1e6 photons, loss sum (abs_E - target)^2",
+                    "form": "fp64 path replay", "histories": n},
+                   forward_histories_per_s=n / (np.mean(fwd_ms) * 1e-3),
+                   adjoint_over_forward_time=float(np.mean(adj_ms) / np.mean(fwd_ms)),
+                   grad_norm={"sigma_a": float(np.linalg.norm(da)),
+                              "fleck": float(np.linalg.norm(df))})
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +278,12 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-chosen", action="store_true",
                     help="skip the chosen form and the history-form comparison")
+    ap.add_argument("--workload", default="cube",
+                    choices=["cube", "hohlraum", "batched", "adjoint"],
+                    help="cube (default): the throughput line; the others measure the "
+                         "30-group hohlraum, the 256-problem batch and the adjoint")
+    ap.add_argument("--fp32-tol", type=float, default=0.0,
+                    help="FP32 tracking: per-deposit relative error bound (0 = default)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: every run "
@@ -164,6 +295,12 @@ def main(argv=None):
 
     from branson_amd import capi
     import numpy as np
+
+    if a.workload != "cube":
+        fn = {"hohlraum": _bench_hohlraum, "batched": _bench_batched,
+              "adjoint": _bench_adjoint}[a.workload]
+        print(json.dumps(fn(a, capi)), flush=True)
+        return
 
     # streaming-copy probe: the HBM rate this device reaches with a plain
     # copy kernel (k_copy_probe, 1 GiB each way), also the calibration kernel
@@ -185,17 +322,15 @@ def main(argv=None):
                    "ms_per_step": 1e3 * hi["dev_s"] / a.steps,
                    "transport_kernel_ms_mean": _roofline("fp64", hi)[1]}
         ch = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP32_TRACK,
-                       capi.ALGO_PERSISTENT, keep_step1=True)
-        p1, c1 = plain["step1"], ch["step1"]
-        m = p1 > 0
-        err = float(np.max(np.abs(c1[m] - p1[m]) / p1[m])) if m.any() else 0.0
+                       capi.ALGO_PERSISTENT, keep_step1=True, fp32_tol=a.fp32_tol)
         cb, cms, cach = _roofline("fp32", ch)
         chosen = {
             "form": "fp32 tracking + fp64 energies and tallies, persistent waves",
             "histories_per_s": ch["hist"] / ch["dev_s"],
             "segments_per_s": ch["segs"] / ch["dev_s"],
             "ms_per_step": 1e3 * ch["dev_s"] / a.steps,
-            "step1_max_rel_abs_E_vs_fp64": err,
+            "step1_tally_error_vs_fp64": _tally_error(plain["step1"], ch["step1"]),
+            "replayed_histories": ch["replayed"],
             "transport_kernel_ms_mean": cms,
             "transport_kernel_GBps_algorithmic": cach,
         }

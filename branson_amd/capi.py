@@ -34,8 +34,10 @@ assert PHOTON_DTYPE.itemsize == 120
 # every symbol include/branson_hip.h declares
 EXPORTS = [
     "bh_abi_version", "bh_create", "bh_destroy", "bh_last_error", "bh_sync", "bh_set_mesh",
-    "bh_set_source_energy", "bh_source", "bh_source_from_energy", "bh_transport",
-    "bh_get_tallies", "bh_census_size", "bh_census_export", "bh_photons_export",
+    "bh_set_source_energy", "bh_source", "bh_source_from_energy", "bh_set_instances",
+    "bh_source_instances", "bh_transport",
+    "bh_step_graph", "bh_get_tallies", "bh_census_size", "bh_census_export",
+    "bh_photons_export",
     "bh_census_import",
     "bh_transport_photons", "bh_adjoint_photons", "bh_rng_draws", "bh_threefry",
     "bh_copy_probe",
@@ -52,7 +54,8 @@ class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("algorithm", C.c_int32), ("block_size", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("tally_mode", C.c_int32),
                 ("refill_threshold", C.c_int32), ("reserve_photons", C.c_uint64),
-                ("fp32_tol", C.c_double)]
+                ("fp32_tol", C.c_double), ("batch_photons", C.c_int32),
+                ("step_graph", C.c_int32)]
 
 
 class Cells(C.Structure):
@@ -96,7 +99,10 @@ def lib() -> C.CDLL:
     L.bh_set_source_energy.argtypes = [P, P, P, P, P]
     L.bh_source.argtypes = [P, U64, D, U32, U32, U32, D, P]
     L.bh_source_from_energy.argtypes = [P, P, P, P, P, U64, D, U32, U32, U32, D, P]
+    L.bh_set_instances.argtypes = [P, U32, P, P]
+    L.bh_source_instances.argtypes = [P, P, P, U32, U32, D, P]
     L.bh_transport.argtypes = [P, D, P, P, P]
+    L.bh_step_graph.argtypes = [P, U64, D, U32, U32, U32, D, D, P, P, P]
     L.bh_get_tallies.argtypes = [P, P, P]
     L.bh_census_size.argtypes = [P, P]
     L.bh_census_export.argtypes = [P, P, U64, P]
@@ -123,11 +129,12 @@ class Engine:
 
     def __init__(self, device: int = 0, precision: int = PRECISION_FP64,
                  algorithm: int = ALGO_HISTORY, tally_mode: int = 0, blocks_per_cu: int = 0,
-                 refill_threshold: int = 0, reserve_photons: int = 0, fp32_tol: float = 0.0):
+                 refill_threshold: int = 0, reserve_photons: int = 0, fp32_tol: float = 0.0,
+                 block_size: int = 0, batch_photons: int = 0):
         L = lib()
         self._h = C.c_void_p()
-        self.opts = Options(precision, algorithm, 0, blocks_per_cu, tally_mode,
-                            refill_threshold, reserve_photons, fp32_tol)
+        self.opts = Options(precision, algorithm, block_size, blocks_per_cu, tally_mode,
+                            refill_threshold, reserve_photons, fp32_tol, batch_photons, 0)
         rc = L.bh_create(device, C.byref(self.opts), C.byref(self._h))
         if rc != BH_OK:
             msg = L.bh_last_error(self._h).decode() if self._h else "bh_create failed"
@@ -189,6 +196,42 @@ class Engine:
         self._check(lib().bh_source_from_energy(self._h, _p(e), _p(s), _p(c), _p(f), n_user,
                                                 total_E, seed, cycle, rank, dt,
                                                 C.byref(n_new)))
+        self._n_new = n_new.value
+        return n_new.value
+
+    def step_graph(self, n_user, total_E, seed, cycle, rank, dt, next_dt):
+        """bh_step_graph: source + transport + census as one graph launch."""
+        st = StepStats()
+        a, t = np.zeros(self.n_cells), np.zeros(self.n_cells)
+        self._check(lib().bh_step_graph(self._h, n_user, total_E, seed, cycle, rank, dt, next_dt,
+                                        _p(a), _p(t), C.byref(st)))
+        self._n_new = 0
+        return a, t, st.as_dict()
+
+    def set_instances(self, first_cell, seeds):
+        """Instances of the mesh: cells [first_cell[k], first_cell[k+1]) with
+        their own seeds (bh_set_instances); empty first_cell clears them."""
+        fc = np.ascontiguousarray(first_cell, dtype=np.uint32)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+        k = max(len(fc) - 1, 0)
+        if k and len(sd) != k:
+            raise ValueError("one seed per instance")
+        self._inst_keep = (fc, sd)
+        self._check(lib().bh_set_instances(self._h, k, _p(fc) if k else None,
+                                           _p(sd) if k else None))
+
+    def source_instances(self, emission_E, source_E, census_E, source_face, n_user, total_E,
+                         cycle, rank, dt) -> int:
+        e = np.ascontiguousarray(emission_E, dtype=np.float64)
+        s = np.ascontiguousarray(source_E, dtype=np.float64)
+        c = None if census_E is None else np.ascontiguousarray(census_E, dtype=np.float64)
+        f = np.ascontiguousarray(source_face, dtype=np.int8)
+        self._check(lib().bh_set_source_energy(self._h, _p(e), _p(s), _p(c), _p(f)))
+        nu = np.ascontiguousarray(n_user, dtype=np.uint64)
+        te = np.ascontiguousarray(total_E, dtype=np.float64)
+        n_new = C.c_uint64()
+        self._check(lib().bh_source_instances(self._h, _p(nu), _p(te), cycle, rank, dt,
+                                              C.byref(n_new)))
         self._n_new = n_new.value
         return n_new.value
 
@@ -350,11 +393,13 @@ class Driver:
                  n_ranks: int = 1, n_photons: int = 0, t_stop: float = 0.0,
                  precision: int = PRECISION_FP64, algorithm: int = ALGO_HISTORY,
                  tally_mode: int = 0, blocks_per_cu: int = 0, refill_threshold: int = 0,
-                 reserve_photons: int = 0, fp32_tol: float = 0.0):
+                 reserve_photons: int = 0, fp32_tol: float = 0.0, block_size: int = 0,
+                 batch_photons: int = 0, step_graph: bool = False):
         L = _host_lib()
         self._d = C.c_void_p()
-        cfg = DriverConfig(Options(precision, algorithm, 0, blocks_per_cu, tally_mode,
-                                   refill_threshold, reserve_photons, fp32_tol), device, n_groups,
+        cfg = DriverConfig(Options(precision, algorithm, block_size, blocks_per_cu, tally_mode,
+                                   refill_threshold, reserve_photons, fp32_tol, batch_photons,
+                                   int(step_graph)), device, n_groups,
                            rank, n_ranks, n_photons, t_stop)
         rc = L.bh_driver_create(os.fsencode(xml_path), C.byref(cfg), C.byref(self._d))
         if rc != BH_OK:

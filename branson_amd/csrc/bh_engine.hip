@@ -110,6 +110,15 @@ struct bh_handle {
   DArr<CensusPartial> parts;
   DArr<AosPhoton> aos;
   DArr<double> adj_g, grad_sa, grad_f;  // adjoint: dL/d abs_E in, gradients out
+
+  // several independent problems in one mesh (n_inst = 0: one problem)
+  uint32_t n_inst = 0;
+  std::vector<uint32_t> inst_first;     // n_inst + 1 cell bounds
+  std::vector<uint64_t> inst_hi;        // seed << 32 per instance
+  std::vector<uint32_t> cell_inst_host;
+  DArr<uint32_t> cell_inst;
+  DArr<uint64_t> inst_ctr_hi;
+  DArr<SourceInst> inst_src;            // [2 * n_inst]: census, then new photons
   DArr<uint64_t> rep_idx;    // FP32 form: histories handed to the FP64 replay
   DArr<uint32_t> rep_first;
   DArr<double> rep_E;
@@ -118,6 +127,20 @@ struct bh_handle {
   DevCounters* d_ctrs = nullptr;
   PersistentQueue* d_queue = nullptr;
   void* h_pinned = nullptr;  // small readbacks
+
+  // the device phase of a step captured as HIP graphs (bh_step_graph), one
+  // per photon-store parity (W and C swap every step)
+  struct StepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> key;  // the buffers and sizes it was captured with
+  };
+  StepGraph graphs[2];
+  StepParams* d_params = nullptr;  // what the graph's kernels read
+  StepParams* h_params = nullptr;  // pinned; the graph's first node copies it in
+  double* h_tally = nullptr;       // pinned: abs_E then track_E
+  size_t h_tally_cap = 0;
+  std::vector<double> em_host, src_host;  // the step's energies, for exact counts
 
   hipEvent_t ev[6] = {};
   double last_source_ms = 0.0;
@@ -200,23 +223,31 @@ int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
 
 DevCells dev_cells(const bh_handle* h) {
   const double tol = h->opt.fp32_tol > 0.0 ? h->opt.fp32_tol : 1e-6;
-  return DevCells{h->geo.p, h->sigma.p, h->next.p, h->n_cells, h->G, (float)tol};
+  const bool inst = h->n_inst > 0;
+  return DevCells{h->geo.p,        h->sigma.p, h->next.p,
+                  h->n_cells,      h->G,       (float)tol,
+                  inst ? h->cell_inst.p : nullptr, inst ? h->inst_ctr_hi.p : nullptr};
 }
 
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)div_up(n, block); }
 
 // Exclusive scan of n uint64 values on the device; the total lands in
 // h->d_total[slot].
-int scan_u64(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out, int slot) {
+// Exclusive scan of in[0, n) into out, total into d_total[slot].  With
+// n_ptr the count is read on the device and n only bounds it (buffers and
+// grid are sized for n).
+int scan_u64(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out, int slot,
+             const uint64_t* n_ptr = nullptr) {
   const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kScanTile), 1);
   // the single-block middle pass loops over the tile sums; keep it short
   if (n_tiles > 65536)
     return fail(h, BH_ERR_ARG, "scan of %llu elements exceeds the supported size",
                 (unsigned long long)n);
   TRY(ensure(h, h->tiles, n_tiles));
-  k_scan_reduce<<<(unsigned)n_tiles, kScanNT, 0, h->stream>>>(in, n, h->tiles.p);
-  k_scan_top<<<1, kScanNT, 0, h->stream>>>(h->tiles.p, n_tiles, h->d_total + slot);
-  k_scan_down<<<(unsigned)n_tiles, kScanNT, 0, h->stream>>>(in, n, h->tiles.p, out);
+  const unsigned grid = (unsigned)std::min<uint64_t>(n_tiles, 4096);
+  k_scan_reduce<<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p);
+  k_scan_top<<<1, kScanNT, 0, h->stream>>>(h->tiles.p, n, n_ptr, h->d_total + slot);
+  k_scan_down<<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p, out);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
 }
@@ -287,10 +318,13 @@ int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const
 }
 
 // Transport of W[0, n) with the configured kernel; counters in h->d_ctrs.
-int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n) {
+// With P (a step being captured as a graph) the persistent kernel reads n on
+// the device and n only sizes the grid and buffers.
+int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n,
+                     const StepParams* P = nullptr) {
   if (n == 0) return BH_OK;
   const DevCells cells = dev_cells(h);
-  const int block = 256;
+  const int block = h->opt.block_size;
   const bool persistent = h->opt.algorithm == BH_ALGO_PERSISTENT;
   const bool lds = use_lds_tally(h, persistent);
   const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
@@ -321,6 +355,7 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n) {
   }
   const int bpc = h->opt.blocks_per_cu > 0 ? h->opt.blocks_per_cu : 8;
   const int refill = h->opt.refill_threshold > 0 ? std::min(h->opt.refill_threshold, kWave) : 16;
+  const uint32_t chunk = h->opt.batch_photons > 0 ? (uint32_t)h->opt.batch_photons : 256u;
   const unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)h->n_cu * bpc,
                                                      std::max<uint64_t>(div_up(n, block), 1));
   k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_queue),
@@ -329,12 +364,12 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n) {
     using T = decltype(t);
     if (fp32) {
       k_transport_persistent<T, float><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, rl);
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
       HIPCHK(h, hipGetLastError());
       return launch_replay(h, cells, W, rl, t, shm);
     }
     k_transport_persistent<T, double><<<grid, block, shm, h->stream>>>(
-        cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, rl);
+        cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   });
@@ -371,8 +406,15 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
   if (!h) return BH_ERR_ALLOC;
   if (opts) h->opt = *opts;
   if (h->opt.block_size == 0) h->opt.block_size = 256;
-  if (h->opt.block_size != 256) {
-    int r = fail(h, BH_ERR_ARG, "block_size %d not supported (256)", h->opt.block_size);
+  if (h->opt.block_size != 64 && h->opt.block_size != 128 && h->opt.block_size != 256) {
+    int r = fail(h, BH_ERR_ARG, "block_size %d not supported (64, 128 or 256)",
+                 h->opt.block_size);
+    *out = h;
+    return r;
+  }
+  if (h->opt.batch_photons < 0 || h->opt.batch_photons > (1 << 20)) {
+    int r = fail(h, BH_ERR_ARG, "batch_photons %d out of range (0..2^20)",
+                 h->opt.batch_photons);
     *out = h;
     return r;
   }
@@ -432,6 +474,9 @@ void bh_destroy(bh_handle* h) {
   release(h->tiles2);
   release(h->parts);
   release(h->aos);
+  release(h->cell_inst);
+  release(h->inst_ctr_hi);
+  release(h->inst_src);
   release(h->adj_g);
   release(h->grad_sa);
   release(h->grad_f);
@@ -441,6 +486,13 @@ void bh_destroy(bh_handle* h) {
   void* ptrs[] = {h->W.slab, h->C.slab, h->d_total, h->d_ctot, h->d_ctrs, h->d_queue};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto& g : h->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  if (h->d_params) (void)hipFree(h->d_params);
+  if (h->h_params) (void)hipHostFree(h->h_params);
+  if (h->h_tally) (void)hipHostFree(h->h_tally);
   if (h->h_pinned) (void)hipHostFree(h->h_pinned);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
@@ -497,6 +549,7 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
     h->has_energy = false;  // energies were sized for the old mesh
     h->n_cen = 0;           // and a resident census indexes the old cells
     h->n_new = 0;
+    h->n_inst = 0;          // as did the instance table
   }
   h->n_cells = n;
   h->G = G;
@@ -537,8 +590,57 @@ int bh_set_source_energy(bh_handle* h, const double* emission_E, const double* s
                              h->stream));
   HIPCHK(h, hipMemcpyAsync(h->face.p, source_face, n, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->em_host.assign(emission_E, emission_E + n);
+  h->src_host.assign(source_E, source_E + n);
   h->has_energy = true;
   h->has_census_E = census_E != nullptr;
+  return BH_OK;
+}
+
+// The step's source for one problem (inst_src null: `census` / `fresh`
+// describe it) or for every instance (inst_src[0, n_inst) census,
+// [n_inst, 2 n_inst) new photons, on the device).
+static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& fresh,
+                      const SourceInst* inst_src, double dt, uint64_t* n_new_out) {
+  const uint32_t n = h->n_cells;
+  const DevCells cells = dev_cells(h);
+  TRY(ensure(h, h->counts, n));
+  TRY(ensure(h, h->off, n));
+  const uint32_t* cinst = h->n_inst ? h->cell_inst.p : nullptr;
+  const SourceInst* icen = inst_src;
+  const SourceInst* inew = inst_src ? inst_src + h->n_inst : nullptr;
+  HIPCHK(h, hipEventRecord(h->ev[4], h->stream));
+  if (h->has_census_E) {  // step 1: initial census replaces the resident one
+    k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->cen.p, nullptr, n, census, icen,
+                                                              cinst, 1, h->counts.p);
+    TRY(scan_u64(h, h->counts.p, n, h->off.p, 0));
+    uint64_t n_c = 0;
+    TRY(read_total(h, 0, &n_c));
+    TRY(grow_store(h, h->W, n_c, 0));
+    if (n_c)
+      k_source_fill<<<grid_for(n_c, 256), 256, 0, h->stream>>>(
+          cells, h->cen.p, nullptr, h->face.p, h->off.p, n_c, census, icen, dt, 1, h->W.soa, 0,
+          nullptr);
+    HIPCHK(h, hipGetLastError());
+    h->n_cen = n_c;
+    h->has_census_E = false;
+  }
+  k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, n, fresh, inew,
+                                                            cinst, 0, h->counts.p);
+  TRY(scan_u64(h, h->counts.p, n, h->off.p, 1));
+  uint64_t n_new = 0;
+  TRY(read_total(h, 1, &n_new));
+  TRY(grow_store(h, h->W, h->n_cen + n_new, h->n_cen));
+  if (n_new)
+    k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
+        cells, h->em.p, h->src.p, h->face.p, h->off.p, n_new, fresh, inew, dt, 0, h->W.soa,
+        h->n_cen, nullptr);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->ev[5], h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->last_source_ms = elapsed_ms(h->ev[4], h->ev[5]);
+  h->n_new = n_new;
+  if (n_new_out) *n_new_out = n_new;
   return BH_OK;
 }
 
@@ -547,54 +649,82 @@ int bh_source(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_t se
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh || !h->has_energy)
     return fail(h, BH_ERR_STATE, "bh_source needs bh_set_mesh and bh_set_source_energy");
+  if (h->n_inst)
+    return fail(h, BH_ERR_STATE, "the mesh holds %u instances: use bh_source_instances",
+                h->n_inst);
   if (!(total_E > 0.0)) return fail(h, BH_ERR_ARG, "total_E must be > 0");
   HIPCHK(h, hipSetDevice(h->device));
-  const uint32_t n = h->n_cells;
-  const DevCells cells = dev_cells(h);
-  TRY(ensure(h, h->counts, n));
-  TRY(ensure(h, h->off, n));
   const uint64_t ctr_hi = (uint64_t)seed << 32;
   if (h->n_cen && ctr_hi != h->ctr_hi)
     return fail(h, BH_ERR_ARG, "seed differs from the resident census photons' seed");
   h->ctr_hi = ctr_hi;
   const uint64_t rank_off = n_user_photons * (uint64_t)rank;
-  HIPCHK(h, hipEventRecord(h->ev[4], h->stream));
+  const SourceInst census{n_user_photons, total_E, rank_off, ctr_hi, 0, 0};
+  const SourceInst fresh{n_user_photons, total_E, 10000000000000ull * (uint64_t)cycle + rank_off,
+                         ctr_hi, 0, 0};
+  return run_source(h, census, fresh, nullptr, dt, n_new_out);
+}
 
-  if (h->has_census_E) {  // step 1: initial census replaces the resident one
-    k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->cen.p, nullptr, n,
-                                                              n_user_photons, total_E, 1,
-                                                              h->counts.p);
-    TRY(scan_u64(h, h->counts.p, n, h->off.p, 0));
-    uint64_t n_c = 0;
-    TRY(read_total(h, 0, &n_c));
-    TRY(grow_store(h, h->W, n_c, 0));
-    if (n_c)
-      k_source_fill<<<grid_for(n_c, 256), 256, 0, h->stream>>>(
-          cells, h->cen.p, nullptr, h->face.p, h->off.p, n_c, n_user_photons, total_E, dt,
-          rank_off, ctr_hi, 1, h->W.soa, 0);
-    HIPCHK(h, hipGetLastError());
-    h->n_cen = n_c;
-    h->has_census_E = false;
+int bh_set_instances(bh_handle* h, uint32_t n_instances, const uint32_t* first_cell,
+                     const uint32_t* seeds) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_instances before bh_set_mesh");
+  HIPCHK(h, hipSetDevice(h->device));
+  if (n_instances == 0) {  // back to one problem
+    h->n_inst = 0;
+    h->n_cen = 0;
+    h->n_new = 0;
+    return BH_OK;
   }
-  k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, n,
-                                                            n_user_photons, total_E, 0,
-                                                            h->counts.p);
-  TRY(scan_u64(h, h->counts.p, n, h->off.p, 1));
-  uint64_t n_new = 0;
-  TRY(read_total(h, 1, &n_new));
-  TRY(grow_store(h, h->W, h->n_cen + n_new, h->n_cen));
-  const uint64_t stream_base = 10000000000000ull * (uint64_t)cycle + rank_off;
-  if (n_new)
-    k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
-        cells, h->em.p, h->src.p, h->face.p, h->off.p, n_new, n_user_photons, total_E, dt,
-        stream_base, ctr_hi, 0, h->W.soa, h->n_cen);
-  HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipEventRecord(h->ev[5], h->stream));
+  if (!first_cell || !seeds) return BH_ERR_ARG;
+  if (first_cell[0] != 0 || first_cell[n_instances] != h->n_cells)
+    return fail(h, BH_ERR_ARG, "instance cell ranges must start at 0 and end at n_cells (%u)",
+                h->n_cells);
+  for (uint32_t k = 0; k < n_instances; ++k)
+    if (first_cell[k + 1] <= first_cell[k])
+      return fail(h, BH_ERR_ARG, "instance %u has no cells", k);
+  h->inst_first.assign(first_cell, first_cell + n_instances + 1);
+  h->inst_hi.resize(n_instances);
+  for (uint32_t k = 0; k < n_instances; ++k) h->inst_hi[k] = (uint64_t)seeds[k] << 32;
+  h->cell_inst_host.resize(h->n_cells);
+  for (uint32_t k = 0; k < n_instances; ++k)
+    for (uint32_t c = first_cell[k]; c < first_cell[k + 1]; ++c) h->cell_inst_host[c] = k;
+  TRY(ensure(h, h->cell_inst, h->n_cells));
+  TRY(ensure(h, h->inst_ctr_hi, n_instances));
+  TRY(ensure(h, h->inst_src, 2 * (size_t)n_instances));
+  HIPCHK(h, hipMemcpyAsync(h->cell_inst.p, h->cell_inst_host.data(),
+                           h->n_cells * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->inst_ctr_hi.p, h->inst_hi.data(), n_instances * sizeof(uint64_t),
+                           hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->last_source_ms = elapsed_ms(h->ev[4], h->ev[5]);
-  h->n_new = n_new;
-  if (n_new_out) *n_new_out = n_new;
+  h->n_inst = n_instances;
+  h->n_cen = 0;  // a resident census belongs to the previous arrangement
+  h->n_new = 0;
   return BH_OK;
+}
+
+int bh_source_instances(bh_handle* h, const uint64_t* n_user_photons, const double* total_E,
+                        uint32_t cycle, uint32_t rank, double dt, uint64_t* n_new_out) {
+  if (!h || !h->stream || !n_user_photons || !total_E) return BH_ERR_ARG;
+  if (!h->has_mesh || !h->has_energy)
+    return fail(h, BH_ERR_STATE,
+                "bh_source_instances needs bh_set_mesh and bh_set_source_energy");
+  if (!h->n_inst) return fail(h, BH_ERR_STATE, "no instances: call bh_set_instances");
+  HIPCHK(h, hipSetDevice(h->device));
+  const uint32_t K = h->n_inst;
+  std::vector<SourceInst> p(2 * (size_t)K);
+  for (uint32_t k = 0; k < K; ++k) {
+    if (!(total_E[k] > 0.0)) return fail(h, BH_ERR_ARG, "instance %u: total_E must be > 0", k);
+    const uint64_t rank_off = n_user_photons[k] * (uint64_t)rank;
+    p[k] = SourceInst{n_user_photons[k], total_E[k], rank_off, h->inst_hi[k], h->inst_first[k], 0};
+    p[K + k] = SourceInst{n_user_photons[k], total_E[k],
+                          10000000000000ull * (uint64_t)cycle + rank_off, h->inst_hi[k],
+                          h->inst_first[k], 0};
+  }
+  HIPCHK(h, hipMemcpyAsync(h->inst_src.p, p.data(), p.size() * sizeof(SourceInst),
+                           hipMemcpyHostToDevice, h->stream));
+  const SourceInst unused{};
+  return run_source(h, unused, unused, h->inst_src.p, dt, n_new_out);
 }
 
 int bh_source_from_energy(bh_handle* h, const double* emission_E, const double* source_E,
@@ -621,14 +751,15 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   TRY(ensure(h, h->tiles2, n_tiles));
   TRY(ensure(h, h->parts, n_tiles));
   TRY(grow_store(h, h->C, n, 0));
-  k_census_count<<<(unsigned)n_tiles, kCensusNT, 0, h->stream>>>(
-      h->W.soa, n, h->n_new, h->n_cen, h->tiles2.p, h->parts.p);
-  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, n_tiles, h->d_ctot);
+  const unsigned cgrid = (unsigned)std::min<uint64_t>(n_tiles, 8192);
+  k_census_count<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, h->n_new, h->n_cen,
+                                                     h->tiles2.p, h->parts.p, nullptr);
+  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, n_tiles, nullptr, h->d_ctot);
   // exclusive scan of per-tile census counts -> tile offsets
   TRY(ensure(h, h->off, n_tiles));
   TRY(scan_u64(h, h->tiles2.p, n_tiles, h->off.p, 2));
-  k_census_scatter<<<(unsigned)n_tiles, kCensusNT, 0, h->stream>>>(
-      h->W.soa, n, h->n_new, h->n_cen, h->off.p, h->C.soa, kC * next_dt);
+  k_census_scatter<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, h->n_new, h->n_cen, h->off.p,
+                                                       h->C.soa, kC * next_dt, nullptr);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
 
@@ -677,6 +808,194 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   return BH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// A step's device phase as one HIP graph launch.
+// ---------------------------------------------------------------------------
+// photons_for of the source kernels, on the host: the same IEEE operations,
+// so the host knows the step's photon count exactly before the launch.
+static inline uint64_t host_photons_for(uint64_t n_user, double E, double total_E) {
+  if (!(E > 0.0)) return 0u;
+  const uint32_t n = (uint32_t)(int)((double)n_user * E / total_E);
+  return n ? n : 1u;
+}
+
+// Everything a captured step bakes in: buffer addresses, sizes and the
+// options that choose its kernels.  A change means capturing again.
+static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
+  auto u = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+  return {u(h->W.slab),      u(h->C.slab),    (uintptr_t)h->W.cap,  (uintptr_t)h->C.cap,
+          u(h->geo.p),       u(h->sigma.p),   u(h->next.p),         u(h->em.p),
+          u(h->src.p),       u(h->face.p),    u(h->counts.p),       u(h->off.p),
+          u(h->tiles.p),     u(h->tiles2.p),  u(h->parts.p),        u(h->abs_E.p),
+          u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
+          u(h->h_tally),     (uintptr_t)h->n_cells, (uintptr_t)h->G,
+          (uintptr_t)h->gray_sigma_a, (uintptr_t)h->ctr_hi};
+}
+
+// Records source + transport + census + readback on h->stream (being
+// captured).  Sizes come from h->d_params at run time; `cap` bounds them.
+static int record_step(bh_handle* h, uint64_t cap, uint64_t n_tiles_cap) {
+  const uint32_t nc = h->n_cells;
+  const DevCells cells = dev_cells(h);
+  StepParams* P = h->d_params;
+  HIPCHK(h, hipMemcpyAsync(P, h->h_params, sizeof(StepParams), hipMemcpyHostToDevice,
+                           h->stream));
+  const SourceInst unused{};
+  k_source_counts<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, nc, unused,
+                                                             &P->fresh, nullptr, 0, h->counts.p);
+  TRY(scan_u64(h, h->counts.p, nc, h->off.p, 1));
+  k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
+      cells, h->em.p, h->src.p, h->face.p, h->off.p, 0, unused, &P->fresh, 0.0, 0, h->W.soa, 0,
+      P);
+  HIPCHK(h, hipGetLastError());
+  TRY(zero_tallies_and_counters(h));
+  TRY(launch_transport(h, h->W.soa, cap, P));
+  const unsigned cgrid = (unsigned)std::min<uint64_t>(n_tiles_cap, 8192);
+  k_census_count<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, 0, 0, 0, h->tiles2.p,
+                                                     h->parts.p, P);
+  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, 0, P, h->d_ctot);
+  TRY(scan_u64(h, h->tiles2.p, n_tiles_cap, h->off.p, 2, &P->n_tiles));
+  k_census_scatter<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, 0, 0, 0, h->off.p, h->C.soa,
+                                                       0.0, P);
+  HIPCHK(h, hipGetLastError());
+  char* hp = static_cast<char*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(hp + 256, h->d_ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(hp + 512, h->d_total + 1, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->h_tally, h->abs_E.p, nc * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->h_tally + nc, h->track_E.p, nc * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+  return BH_OK;
+}
+
+int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_t seed,
+                  uint32_t cycle, uint32_t rank, double dt, double next_dt, double* abs_E,
+                  double* track_E, bh_step_stats* stats) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh || !h->has_energy)
+    return fail(h, BH_ERR_STATE, "bh_step_graph needs bh_set_mesh and bh_set_source_energy");
+  if (h->opt.algorithm != BH_ALGO_PERSISTENT)
+    return fail(h, BH_ERR_STATE, "bh_step_graph needs the persistent algorithm");
+  if (h->n_inst) return fail(h, BH_ERR_STATE, "bh_step_graph runs one problem (no instances)");
+  if (!(total_E > 0.0)) return fail(h, BH_ERR_ARG, "total_E must be > 0");
+  if (h->has_census_E) {  // the first step makes the initial census: eager
+    TRY(bh_source(h, n_user_photons, total_E, seed, cycle, rank, dt, nullptr));
+    return bh_transport(h, next_dt, abs_E, track_E, stats);
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  const uint64_t ctr_hi = (uint64_t)seed << 32;
+  if (h->n_cen && ctr_hi != h->ctr_hi)
+    return fail(h, BH_ERR_ARG, "seed differs from the resident census photons' seed");
+  h->ctr_hi = ctr_hi;
+  const uint32_t nc = h->n_cells;
+  uint64_t n_new = 0;
+  for (uint32_t c = 0; c < nc; ++c)
+    n_new += host_photons_for(n_user_photons, h->em_host[c], total_E) +
+             host_photons_for(n_user_photons, h->src_host[c], total_E);
+  const uint64_t n = h->n_cen + n_new;
+  // every buffer the graph touches is sized (and allocated) before capture
+  TRY(grow_store(h, h->W, n, h->n_cen));
+  TRY(grow_store(h, h->C, n, 0));
+  const uint64_t cap = std::max(h->W.cap, h->C.cap);
+  const uint64_t n_tiles_cap = std::max<uint64_t>(div_up(cap, kCensusTile), 1);
+  TRY(ensure(h, h->counts, nc));
+  TRY(ensure(h, h->off, std::max<uint64_t>(nc, n_tiles_cap)));
+  TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(std::max<uint64_t>(nc, n_tiles_cap),
+                                                     kScanTile), 1)));
+  TRY(ensure(h, h->tiles2, n_tiles_cap));
+  TRY(ensure(h, h->parts, n_tiles_cap));
+  if (h->opt.precision == BH_PRECISION_FP32_TRACK) {
+    TRY(ensure(h, h->rep_idx, cap));
+    TRY(ensure(h, h->rep_first, cap));
+    TRY(ensure(h, h->rep_E, cap));
+  }
+  if (!h->d_params) HIPCHK(h, hipMalloc(&h->d_params, sizeof(StepParams)));
+  if (!h->h_params) HIPCHK(h, hipHostMalloc(&h->h_params, sizeof(StepParams), hipHostMallocDefault));
+  if (h->h_tally_cap < 2 * (size_t)nc) {
+    if (h->h_tally) HIPCHK(h, hipHostFree(h->h_tally));
+    h->h_tally = nullptr;
+    h->h_tally_cap = 0;
+    HIPCHK(h, hipHostMalloc(&h->h_tally, 2 * (size_t)nc * sizeof(double), hipHostMallocDefault));
+    h->h_tally_cap = 2 * (size_t)nc;
+  }
+  const uint64_t rank_off = n_user_photons * (uint64_t)rank;
+  *h->h_params = StepParams{h->n_cen, n_new, n,
+                            std::max<uint64_t>(div_up(n, kCensusTile), 1), kC * next_dt, dt,
+                            SourceInst{n_user_photons, total_E,
+                                       10000000000000ull * (uint64_t)cycle + rank_off, ctr_hi,
+                                       0, 0}};
+  // one graph per parity of the photon stores (they swap every step)
+  bh_handle::StepGraph& g = h->graphs[h->W.slab < h->C.slab ? 0 : 1];
+  const std::vector<uintptr_t> key = step_graph_key(h);
+  if (!g.exec || g.key != key) {
+    if (g.exec) HIPCHK(h, hipGraphExecDestroy(g.exec));
+    if (g.graph) HIPCHK(h, hipGraphDestroy(g.graph));
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.key.clear();
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_step(h, cap, n_tiles_cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (rc != BH_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess)
+      return fail(h, BH_ERR_DEVICE, "capturing the step failed: %s", hipGetErrorString(e));
+    g.graph = graph;
+    HIPCHK(h, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    g.key = key;
+  }
+  HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+  HIPCHK(h, hipGraphLaunch(g.exec, h->stream));
+  HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  CensusPartial tot;
+  DevCounters ctrs;
+  uint64_t dev_new = 0;
+  const char* hp = static_cast<const char*>(h->h_pinned);
+  std::memcpy(&tot, hp, sizeof tot);
+  std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  std::memcpy(&dev_new, hp + 512, sizeof dev_new);
+  if (dev_new != n_new) {  // the host and device counts are the same IEEE arithmetic
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE, "step graph: device made %llu photons, host expected %llu",
+                (unsigned long long)dev_new, (unsigned long long)n_new);
+  }
+  if (abs_E) std::memcpy(abs_E, h->h_tally, nc * sizeof(double));
+  if (track_E) std::memcpy(track_E, h->h_tally + nc, nc * sizeof(double));
+  std::swap(h->W, h->C);
+  h->n_cen = tot.n_census;
+  h->n_new = 0;
+  h->last_source_ms = 0.0;
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->census_E = tot.census_E;
+    stats->exit_E = tot.exit_E;
+    stats->n_transported = n;
+    stats->n_census = tot.n_census;
+    stats->n_exit = tot.n_exit;
+    stats->n_killed = tot.n_killed;
+    stats->n_segments = ctrs.segments;
+    stats->n_flushes = ctrs.flushes;
+    stats->n_replayed = ctrs.replays;
+    stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph
+  }
+  if (ctrs.stuck)
+    return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
+                ctrs.stuck, kMaxSegments);
+  if (tot.n_other)
+    return fail(h, BH_ERR_STATE, "%llu photon(s) ended without EXIT/CENSUS/KILLED",
+                tot.n_other);
+  return BH_OK;
+}
+
 int bh_get_tallies(bh_handle* h, double* abs_E, double* track_E) {
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "no mesh");
@@ -706,7 +1025,8 @@ int bh_census_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
   HIPCHK(h, hipSetDevice(h->device));
   TRY(ensure(h, h->aos, h->n_cen));
   k_soa_to_aos<<<grid_for(h->n_cen, 256), 256, 0, h->stream>>>(h->W.soa, 0, h->n_cen,
-                                                                h->ctr_hi, h->aos.p);
+                                                                h->ctr_hi, dev_cells(h),
+                                                                h->aos.p);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, h->aos.p, h->n_cen * sizeof(AosPhoton), hipMemcpyDeviceToHost,
                            h->stream));
@@ -727,10 +1047,12 @@ int bh_photons_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
   // new photons live at W[n_cen, n_cen + n_new), census at W[0, n_cen)
   if (h->n_new)
     k_soa_to_aos<<<grid_for(h->n_new, 256), 256, 0, h->stream>>>(h->W.soa, h->n_cen, h->n_new,
-                                                                  h->ctr_hi, h->aos.p);
+                                                                  h->ctr_hi, dev_cells(h),
+                                                                  h->aos.p);
   if (h->n_cen)
     k_soa_to_aos<<<grid_for(h->n_cen, 256), 256, 0, h->stream>>>(h->W.soa, 0, h->n_cen,
-                                                                  h->ctr_hi, h->aos.p + h->n_new);
+                                                                  h->ctr_hi, dev_cells(h),
+                                                                  h->aos.p + h->n_new);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, h->aos.p, total * sizeof(AosPhoton), hipMemcpyDeviceToHost,
                            h->stream));
@@ -743,17 +1065,21 @@ static int upload_aos(bh_handle* h, const bh_photon* in, uint64_t n, PhotonStore
   if (n == 0) return BH_OK;
   const uint64_t hi = in[0].rng[1];
   for (uint64_t i = 0; i < n; ++i) {
-    if (in[i].rng[1] != hi || in[i].rng[3] != 0)
-      return fail(h, BH_ERR_ARG,
-                  "photon %llu: RNG counter-high/key-high words differ from photon 0 "
-                  "(one seed per run is required)",
-                  (unsigned long long)i);
     if (h->has_mesh && in[i].cell >= h->n_cells)
       return fail(h, BH_ERR_ARG, "photon %llu: cell %u out of range",
                   (unsigned long long)i, in[i].cell);
     if (h->has_mesh && in[i].group >= h->G)
       return fail(h, BH_ERR_ARG, "photon %llu: group %u out of range",
                   (unsigned long long)i, in[i].group);
+    // the device keeps one counter-high word per run (or per instance)
+    const uint64_t want = h->n_inst ? h->inst_hi[h->cell_inst_host[in[i].cell]] : hi;
+    if (in[i].rng[1] != want || in[i].rng[3] != 0)
+      return fail(h, BH_ERR_ARG,
+                  h->n_inst ? "photon %llu: RNG counter-high/key-high words differ from its "
+                              "instance's seed"
+                            : "photon %llu: RNG counter-high/key-high words differ from "
+                              "photon 0 (one seed per run is required)",
+                  (unsigned long long)i);
   }
   h->ctr_hi = hi;
   TRY(ensure(h, h->aos, n));
@@ -788,7 +1114,8 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
   TRY(launch_transport(h, h->C.soa, n));
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-  k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(h->C.soa, 0, n, h->ctr_hi, h->aos.p);
+  k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(h->C.soa, 0, n, h->ctr_hi, dev_cells(h),
+                                                        h->aos.p);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(photons, h->aos.p, n * sizeof(AosPhoton), hipMemcpyDeviceToHost,
                            h->stream));

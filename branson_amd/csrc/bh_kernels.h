@@ -30,6 +30,39 @@ struct DevCells {
   uint32_t n_cells;
   uint32_t G;
   float fp32_tol;         // FP32 tracking: largest relative error bound of a deposit
+  // several independent problems in one mesh (else null): the instance of
+  // each cell and each instance's RNG counter-high word (seed << 32)
+  const uint32_t* cell_inst;
+  const uint64_t* inst_ctr_hi;
+};
+
+// The RNG counter-high word of a photon in cell c.
+__device__ __forceinline__ uint64_t photon_ctr_hi(const DevCells& cells, uint32_t c,
+                                                  uint64_t ctr_hi) {
+  return cells.cell_inst ? cells.inst_ctr_hi[cells.cell_inst[c]] : ctr_hi;
+}
+
+// Source parameters of one problem instance (a single problem is one
+// instance starting at cell 0).
+struct SourceInst {
+  uint64_t n_user;       // photons requested for the instance
+  double total_E;        // the instance's total photon energy
+  uint64_t stream_base;  // 1e13 cycle + n_user rank, or n_user rank (initial census)
+  uint64_t ctr_hi;       // seed << 32
+  uint32_t first_cell;   // the instance's cells are [first_cell, next instance's)
+  uint32_t pad;
+};
+
+// Per-step values a step captured as a HIP graph reads on the device (the
+// graph copies them in from pinned host memory when it starts).
+struct StepParams {
+  uint64_t n_cen;       // resident census photons, stored first in W
+  uint64_t n_new;       // the step's new photons, stored after them
+  uint64_t n;           // n_cen + n_new
+  uint64_t n_tiles;     // census compaction tiles of n (at least 1)
+  double census_life;   // c * next_dt
+  double dt;
+  SourceInst fresh;     // one problem's source parameters
 };
 
 struct DevCounters {
@@ -113,27 +146,40 @@ constexpr int kScanNT = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanNT * kScanItems;
 
-__global__ __launch_bounds__(kScanNT) void k_scan_reduce(const uint64_t* __restrict__ in,
-                                                         uint64_t n,
-                                                         uint64_t* __restrict__ tile_sum) {
-  __shared__ uint64_t lds[kScanNT / kWave];
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  uint64_t s = 0;
-#pragma unroll
-  for (int it = 0; it < kScanItems; ++it) {
-    const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
-    if (i < n) s += in[i];
-  }
-  s = block_sum<uint64_t, kScanNT>(s, lds);
-  if (threadIdx.x == 0) tile_sum[blockIdx.x] = s;
+// Three-pass exclusive scan of n items.  n is the argument, or *n_ptr when
+// n_ptr is set (sizes that only the device knows, e.g. inside a captured
+// graph); the tile loops are grid-stride, so any grid works.
+__device__ __forceinline__ uint64_t scan_n(uint64_t n, const uint64_t* n_ptr) {
+  return n_ptr ? *n_ptr : n;
 }
 
-// One block of 256 threads scans n_tiles sums in place (exclusive) and
+__global__ __launch_bounds__(kScanNT) void k_scan_reduce(const uint64_t* __restrict__ in,
+                                                         uint64_t n, const uint64_t* n_ptr,
+                                                         uint64_t* __restrict__ tile_sum) {
+  __shared__ uint64_t lds[kScanNT / kWave];
+  n = scan_n(n, n_ptr);
+  const uint64_t n_tiles = (n + kScanTile - 1) / kScanTile;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint64_t base = t * kScanTile;
+    uint64_t s = 0;
+#pragma unroll
+    for (int it = 0; it < kScanItems; ++it) {
+      const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
+      if (i < n) s += in[i];
+    }
+    s = block_sum<uint64_t, kScanNT>(s, lds);
+    if (threadIdx.x == 0) tile_sum[t] = s;
+  }
+}
+
+// One block of 256 threads scans the tile sums in place (exclusive) and
 // writes the grand total to *total.
 __global__ __launch_bounds__(kScanNT) void k_scan_top(uint64_t* __restrict__ tile_sum,
-                                                      uint64_t n_tiles,
+                                                      uint64_t n, const uint64_t* n_ptr,
                                                       uint64_t* __restrict__ total) {
   __shared__ uint64_t lds[kScanNT / kWave];
+  n = scan_n(n, n_ptr);
+  const uint64_t n_tiles = (n + kScanTile - 1) / kScanTile;
   uint64_t carry = 0;
   for (uint64_t b = 0; b < n_tiles; b += kScanNT) {
     const uint64_t i = b + threadIdx.x;
@@ -147,20 +193,24 @@ __global__ __launch_bounds__(kScanNT) void k_scan_top(uint64_t* __restrict__ til
 }
 
 __global__ __launch_bounds__(kScanNT) void k_scan_down(const uint64_t* __restrict__ in,
-                                                       uint64_t n,
+                                                       uint64_t n, const uint64_t* n_ptr,
                                                        const uint64_t* __restrict__ tile_off,
                                                        uint64_t* __restrict__ out) {
   __shared__ uint64_t lds[kScanNT / kWave];
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  uint64_t carry = tile_off[blockIdx.x];
+  n = scan_n(n, n_ptr);
+  const uint64_t n_tiles = (n + kScanTile - 1) / kScanTile;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint64_t base = t * kScanTile;
+    uint64_t carry = tile_off[t];
 #pragma unroll
-  for (int it = 0; it < kScanItems; ++it) {
-    const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
-    const uint64_t v = (i < n) ? in[i] : 0;
-    uint64_t tot;
-    const uint64_t ex = block_excl_scan<uint64_t, kScanNT>(v, lds, tot);
-    if (i < n) out[i] = carry + ex;
-    carry += tot;
+    for (int it = 0; it < kScanItems; ++it) {
+      const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
+      const uint64_t v = (i < n) ? in[i] : 0;
+      uint64_t tot;
+      const uint64_t ex = block_excl_scan<uint64_t, kScanNT>(v, lds, tot);
+      if (i < n) out[i] = carry + ex;
+      carry += tot;
+    }
   }
 }
 
@@ -188,27 +238,57 @@ __device__ __forceinline__ uint32_t photons_for(uint64_t n_user, double E, doubl
 // census (mode 1, emission_E then holds the census energy).
 __global__ void k_source_counts(const double* __restrict__ emission_E,
                                 const double* __restrict__ source_E, uint32_t n_cells,
-                                uint64_t n_user, double total_E, int mode,
+                                SourceInst one, const SourceInst* __restrict__ inst,
+                                const uint32_t* __restrict__ cell_inst, int mode,
                                 uint64_t* __restrict__ counts) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cells) return;
-  uint64_t n = photons_for(n_user, emission_E[c], total_E);
-  if (mode == 0) n += photons_for(n_user, source_E[c], total_E);
+  const SourceInst sp = inst ? inst[cell_inst ? cell_inst[c] : 0u] : one;
+  uint64_t n = photons_for(sp.n_user, emission_E[c], sp.total_E);
+  if (mode == 0) n += photons_for(sp.n_user, source_E[c], sp.total_E);
   counts[c] = n;
 }
 
 // One work-item per new photon p: locate its cell in the exclusive prefix
-// `off` (last cell with off[c] <= p), then sample it.  Stream number is
-// stream_base + p: 1e13*cycle + n_user*rank + p for new photons
-// (src/source.h:221-222, 267, 283), n_user*rank + p for the initial census
-// (src/source.h:144, 171).  Output goes to slot out_base + p of W.
+// `off` (last cell with off[c] <= p), then sample it.  Stream number is the
+// instance's stream_base + its photon index within the instance:
+// 1e13*cycle + n_user*rank + i for new photons (src/source.h:221-222, 267,
+// 283), n_user*rank + i for the initial census (src/source.h:144, 171).
+// Output goes to slot out_base + p of W.
+// With `P` set (a whole step captured as a graph), n_new, out_base and dt
+// are read on the device.  Grid-stride loop.
+__device__ __forceinline__ void source_one(const DevCells& cells,
+                                           const double* __restrict__ emission_E,
+                                           const double* __restrict__ source_E,
+                                           const int8_t* __restrict__ face_of,
+                                           const uint64_t* __restrict__ off, uint64_t p,
+                                           const SourceInst& one,
+                                           const SourceInst* __restrict__ inst, double dt,
+                                           int mode, const PhotonSoA& W, uint64_t out_base);
+
 __global__ __launch_bounds__(256) void k_source_fill(
     DevCells cells, const double* __restrict__ emission_E, const double* __restrict__ source_E,
     const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off, uint64_t n_new,
-    uint64_t n_user, double total_E, double dt, uint64_t stream_base, uint64_t ctr_hi,
-    int mode, PhotonSoA W, uint64_t out_base) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_new) return;
+    SourceInst one, const SourceInst* __restrict__ inst, double dt, int mode, PhotonSoA W,
+    uint64_t out_base, const StepParams* __restrict__ P) {
+  if (P) {
+    out_base = P->n_cen;
+    n_new = P->n_new;
+    dt = P->dt;
+  }
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n_new; p += stride)
+    source_one(cells, emission_E, source_E, face_of, off, p, one, inst, dt, mode, W, out_base);
+}
+
+__device__ __forceinline__ void source_one(const DevCells& cells,
+                                           const double* __restrict__ emission_E,
+                                           const double* __restrict__ source_E,
+                                           const int8_t* __restrict__ face_of,
+                                           const uint64_t* __restrict__ off, uint64_t p,
+                                           const SourceInst& one,
+                                           const SourceInst* __restrict__ inst, double dt,
+                                           int mode, const PhotonSoA& W, uint64_t out_base) {
   uint32_t lo = 0, hi = cells.n_cells;  // upper_bound(off, p) - 1
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -219,10 +299,13 @@ __global__ __launch_bounds__(256) void k_source_fill(
   }
   const uint32_t c = lo - 1;
   const uint64_t local = p - off[c];
+  const SourceInst sp = inst ? inst[cells.cell_inst ? cells.cell_inst[c] : 0u] : one;
+  const uint64_t n_user = sp.n_user;
+  const double total_E = sp.total_E;
   const CellGeo& g = cells.geo[c];
   const double nd[6] = {g.nodes[0], g.nodes[1], g.nodes[2], g.nodes[3], g.nodes[4], g.nodes[5]};
 
-  Rng rng{0ull, stream_base + p, ctr_hi};
+  Rng rng{0ull, sp.stream_base + (p - off[sp.first_cell]), sp.ctr_hi};
   double x, y, z, u, v, w, E, life;
   uint8_t stype;
   const uint32_t n_em = photons_for(n_user, emission_E[c], total_E);
@@ -432,13 +515,39 @@ __device__ __forceinline__ uint64_t logical_to_phys(uint64_t k, uint64_t n_new, 
   return (k < n_new) ? n_cen + k : k - n_new;
 }
 
+// The census kernels loop over tiles (grid-stride); with P set the counts
+// come from the device.
+__device__ __forceinline__ void census_count_tile(const PhotonSoA& W, uint64_t n, uint64_t n_new,
+                                                  uint64_t n_cen, uint64_t tile,
+                                                  uint64_t* __restrict__ tile_count,
+                                                  CensusPartial* __restrict__ part);
+
 __global__ __launch_bounds__(kCensusNT) void k_census_count(PhotonSoA W, uint64_t n,
                                                             uint64_t n_new, uint64_t n_cen,
                                                             uint64_t* __restrict__ tile_count,
-                                                            CensusPartial* __restrict__ part) {
+                                                            CensusPartial* __restrict__ part,
+                                                            const StepParams* __restrict__ P) {
+  if (P) {
+    n = P->n;
+    n_new = P->n_new;
+    n_cen = P->n_cen;
+  }
+  const uint64_t n_tiles = (n + kCensusTile - 1) / kCensusTile;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x)
+    census_count_tile(W, n, n_new, n_cen, t, tile_count, part);
+  if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {  // one empty tile
+    tile_count[0] = 0;
+    part[0] = CensusPartial{0.0, 0.0, 0, 0, 0, 0};
+  }
+}
+
+__device__ __forceinline__ void census_count_tile(const PhotonSoA& W, uint64_t n, uint64_t n_new,
+                                                  uint64_t n_cen, uint64_t tile,
+                                                  uint64_t* __restrict__ tile_count,
+                                                  CensusPartial* __restrict__ part) {
   __shared__ double ldsd[kCensusNT / kWave];
   __shared__ unsigned long long ldsu[kCensusNT / kWave];
-  const uint64_t base = (uint64_t)blockIdx.x * kCensusTile;
+  const uint64_t base = tile * kCensusTile;
   double cE = 0.0, xE = 0.0;
   unsigned long long nc = 0, nx = 0, nk = 0, no = 0;
 #pragma unroll
@@ -467,15 +576,17 @@ __global__ __launch_bounds__(kCensusNT) void k_census_count(PhotonSoA W, uint64_
   nk = block_sum<unsigned long long, kCensusNT>(nk, ldsu);
   no = block_sum<unsigned long long, kCensusNT>(no, ldsu);
   if (threadIdx.x == 0) {
-    tile_count[blockIdx.x] = nc;
-    part[blockIdx.x] = CensusPartial{cE, xE, nc, nx, nk, no};
+    tile_count[tile] = nc;
+    part[tile] = CensusPartial{cE, xE, nc, nx, nk, no};
   }
 }
 
 // Single block: deterministic in-order reduction of the per-tile partials.
 __global__ __launch_bounds__(256) void k_census_totals(const CensusPartial* __restrict__ part,
                                                        uint64_t n_tiles,
+                                                       const StepParams* __restrict__ P,
                                                        CensusPartial* __restrict__ out) {
+  if (P) n_tiles = P->n_tiles;
   __shared__ double ldsd[4];
   __shared__ unsigned long long ldsu[4];
   double cE = 0.0, xE = 0.0;
@@ -500,12 +611,35 @@ __global__ __launch_bounds__(256) void k_census_totals(const CensusPartial* __re
   if (threadIdx.x == 0) *out = CensusPartial{cE, xE, nc, nx, nk, no};
 }
 
+__device__ __forceinline__ void census_scatter_tile(const PhotonSoA& W, uint64_t n,
+                                                    uint64_t n_new, uint64_t n_cen,
+                                                    uint64_t tile,
+                                                    const uint64_t* __restrict__ tile_off,
+                                                    const PhotonSoA& C, double census_life);
+
 __global__ __launch_bounds__(kCensusNT) void k_census_scatter(
     PhotonSoA W, uint64_t n, uint64_t n_new, uint64_t n_cen,
-    const uint64_t* __restrict__ tile_off, PhotonSoA C, double census_life) {
+    const uint64_t* __restrict__ tile_off, PhotonSoA C, double census_life,
+    const StepParams* __restrict__ P) {
+  if (P) {
+    n = P->n;
+    n_new = P->n_new;
+    n_cen = P->n_cen;
+    census_life = P->census_life;
+  }
+  const uint64_t n_tiles = (n + kCensusTile - 1) / kCensusTile;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x)
+    census_scatter_tile(W, n, n_new, n_cen, t, tile_off, C, census_life);
+}
+
+__device__ __forceinline__ void census_scatter_tile(const PhotonSoA& W, uint64_t n,
+                                                    uint64_t n_new, uint64_t n_cen,
+                                                    uint64_t tile,
+                                                    const uint64_t* __restrict__ tile_off,
+                                                    const PhotonSoA& C, double census_life) {
   __shared__ uint32_t lds[kCensusNT / kWave];
-  const uint64_t base = (uint64_t)blockIdx.x * kCensusTile;
-  uint64_t carry = tile_off[blockIdx.x];
+  const uint64_t base = tile * kCensusTile;
+  uint64_t carry = tile_off[tile];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
 #pragma unroll
@@ -587,12 +721,15 @@ __global__ void k_aos_to_soa(const AosPhoton* __restrict__ a, uint64_t n, Photon
 }
 
 __global__ void k_soa_to_aos(PhotonSoA W, uint64_t base, uint64_t n, uint64_t ctr_hi,
-                             AosPhoton* __restrict__ a) {
+                             DevCells cells, AosPhoton* __restrict__ a) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t s = base + i;
   AosPhoton p;
   p.cell = W.cell[s];
+  // per-instance seed; a photon that left through a PROCESSOR face may name
+  // a cell of another mesh, which keeps the run's word
+  if (p.cell < cells.n_cells) ctr_hi = photon_ctr_hi(cells, p.cell, ctr_hi);
   p.group = W.group[s];
   p.source_type = W.stype[s];
   p.descriptor[0] = W.desc[s];

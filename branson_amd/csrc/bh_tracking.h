@@ -109,7 +109,7 @@ __device__ __forceinline__ void load_live(const DevCells& cells, const PhotonSoA
   L.E0 = W.E0[i];
   L.rng.ctr = W.ctr[i];
   L.rng.stream = W.stream[i];
-  L.rng.ctr_hi = ctr_hi;
+  L.rng.ctr_hi = photon_ctr_hi(cells, L.cell, ctr_hi);
   L.group = W.group[i];
   L.desc = W.desc[i];
   L.sc = 0;  // surface_cross starts at 0 for every photon (:43)
@@ -580,14 +580,21 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
   return true;
 }
 
+// Wave-level sums, one atomic per wave and counter: valid for any block
+// size (the tracking kernels run with 64, 128 or 256 work-items).
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
 __device__ __forceinline__ void add_counters(DevCounters* ctrs, unsigned long long segs,
                                              unsigned long long flushes,
                                              unsigned long long stuck) {
-  __shared__ unsigned long long lds[4];
-  const unsigned long long s = block_sum<unsigned long long, 256>(segs, lds);
-  const unsigned long long f = block_sum<unsigned long long, 256>(flushes, lds);
-  const unsigned long long k = block_sum<unsigned long long, 256>(stuck, lds);
-  if (threadIdx.x == 0) {
+  const unsigned long long s = wave_sum_u64(segs);
+  const unsigned long long f = wave_sum_u64(flushes);
+  const unsigned long long k = wave_sum_u64(stuck);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
     atomicAdd(&ctrs->segments, s);
     atomicAdd(&ctrs->flushes, f);
     if (k) atomicAdd(&ctrs->stuck, k);
@@ -673,15 +680,17 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
 }
 
 // Persistent form: each wave keeps 64 live photons and refills idle lanes
-// from a chunk of photon indices it owns; a chunk of kChunk indices is
+// from a chunk of photon indices it owns; a chunk of `chunk` indices is
 // claimed with one atomic on the global queue head, so the queue sees one
-// atomic per kChunk histories instead of one per refill.  The loop ends
+// atomic per chunk histories instead of one per refill.  The loop ends
 // when the queue is drained and no lane holds a photon, so every wave exits.
 template <class Tally, typename R>
 BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_t n,
                                             uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
-                                            PersistentQueue* q, int refill, ReplayList rl) {
-  constexpr uint64_t kChunk = 256;
+                                            PersistentQueue* q, int refill, uint32_t chunk,
+                                            ReplayList rl, const StepParams* P) {
+  const uint64_t kChunk = chunk;
+  if (P) n = P->n;  // a step captured as a graph
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
   const int lane = threadIdx.x & (kWave - 1);

@@ -505,15 +505,23 @@ int bh_driver_transport(bh_driver* d) {
   if (!(total_E > 0.0)) return dfail(d, BH_ERR_STATE, "no photon energy this step");
   // pre-transport census energy (src/replicated_driver.h:61, 71)
   d->st.pre_census_E = (d->st.step == 1) ? initial_census_energy(d, total_E) : d->prev_census_E;
-  uint64_t n_new = 0;
-  int rc = bh_source(d->eng, d->deck.n_photons, total_E, d->deck.seed, d->st.step, d->cfg.rank,
-                     d->st.dt, &n_new);
-  if (rc != BH_OK) return engine_fail(d, rc);
-  uint64_t n_cen = 0;
-  bh_census_size(d->eng, &n_cen);
-  d->transported = n_new + n_cen;
-  rc = bh_transport(d->eng, d->st.next_dt(), nullptr, nullptr, &d->stats);
-  if (rc != BH_OK) return engine_fail(d, rc);
+  if (d->cfg.engine.step_graph) {  // the whole device phase as one graph launch
+    const int rc = bh_step_graph(d->eng, d->deck.n_photons, total_E, d->deck.seed, d->st.step,
+                                 d->cfg.rank, d->st.dt, d->st.next_dt(), nullptr, nullptr,
+                                 &d->stats);
+    if (rc != BH_OK) return engine_fail(d, rc);
+    d->transported = d->stats.n_transported;
+  } else {
+    uint64_t n_new = 0;
+    int rc = bh_source(d->eng, d->deck.n_photons, total_E, d->deck.seed, d->st.step,
+                       d->cfg.rank, d->st.dt, &n_new);
+    if (rc != BH_OK) return engine_fail(d, rc);
+    uint64_t n_cen = 0;
+    bh_census_size(d->eng, &n_cen);
+    d->transported = n_new + n_cen;
+    rc = bh_transport(d->eng, d->st.next_dt(), nullptr, nullptr, &d->stats);
+    if (rc != BH_OK) return engine_fail(d, rc);
+  }
   d->device_ms = ms_since(t0);
   d->phase = bh_driver::kTransported;
   return BH_OK;

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .capi import ELEMENT, VACUUM
+from .capi import ELEMENT, REFLECT, VACUUM
 
 A_RAD = 0.01372      # GJ / cm^3 / keV^4 (src/constants.h:22)
 C_LIGHT = 299.792458  # cm / sh (src/constants.h:17)
@@ -113,3 +113,60 @@ def adjoint_field_problem(n: int = 32, width: float = 0.05, median: float = 10.0
     nodes, bc, nxt = box_mesh(n, n, n, width, width, width, VACUUM)
     sa = lognormal_field((n, n, n), median, sigma_log, corr_cells, field_seed)
     return CellProblem(nodes, bc, nxt, sa, sigma_s, T, dt)
+
+
+class Batch:
+    """Independent problems concatenated into one mesh (bh_set_instances):
+    instance k owns cells [first_cell[k], first_cell[k+1]), neighbours are
+    global indices, and each instance keeps its own seed, photon count,
+    temperature and opacities."""
+
+    def __init__(self, parts, n_photons, seeds):
+        self.parts = list(parts)
+        self.n_instances = len(self.parts)
+        self.n_photons = np.asarray(n_photons, dtype=np.uint64)
+        self.seeds = np.asarray(seeds, dtype=np.uint32)
+        sizes = [p.n_cells for p in self.parts]
+        self.first_cell = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+        off = self.first_cell[:-1]
+        self.nodes = np.concatenate([p.nodes for p in self.parts])
+        self.bc = np.concatenate([p.bc for p in self.parts])
+        self.next = np.concatenate([p.next.astype(np.int64) + o
+                                    for p, o in zip(self.parts, off)]).astype(np.uint32)
+        self.n_cells = int(self.first_cell[-1])
+
+    def cat(self, attr):
+        return np.concatenate([getattr(p, attr) for p in self.parts])
+
+    def instance_of_cell(self):
+        return np.repeat(np.arange(self.n_instances), np.diff(self.first_cell))
+
+
+def batched_instances(n_instances: int = 256, seed: int = 0, cells_range=(50, 32768),
+                      photons_range=(10**4, 10**6), width: float = 0.02,
+                      dt: float = 0.01) -> Batch:
+    """The batched workload: instances alternate between slabs (n x 1 x 1,
+    reflecting side faces) and cubes (about n^(1/3) cells a side, vacuum),
+    with cell counts and photon counts drawn log-uniform, per-cell sigma_a
+    lognormal (median 10/cm, sigma_log 1.0), sigma_s uniform in [0, 2]/cm, and
+    the instance temperature T0 uniform in [0.01, 1] keV.  Instance k uses
+    seed k for its photons."""
+    rng = np.random.default_rng(seed)
+    lo_c, hi_c = np.log(cells_range[0]), np.log(cells_range[1])
+    lo_p, hi_p = np.log(photons_range[0]), np.log(photons_range[1])
+    parts, n_ph = [], []
+    for k in range(n_instances):
+        nc = int(round(np.exp(rng.uniform(lo_c, hi_c))))
+        if k % 2 == 0:
+            kinds = (VACUUM, VACUUM, REFLECT, REFLECT, REFLECT, REFLECT)
+            nodes, bc, nxt = box_mesh(nc, 1, 1, width, width, width, kinds)
+        else:
+            m = max(2, int(round(nc ** (1.0 / 3.0))))
+            nodes, bc, nxt = box_mesh(m, m, m, width, width, width, VACUUM)
+        n = len(nodes)
+        sa = 10.0 * np.exp(rng.normal(0.0, 1.0, n))
+        ss = rng.uniform(0.0, 2.0, n)
+        T0 = rng.uniform(0.01, 1.0)
+        parts.append(CellProblem(nodes, bc, nxt, sa, ss, T0, dt))
+        n_ph.append(int(round(np.exp(rng.uniform(lo_p, hi_p)))))
+    return Batch(parts, n_ph, np.arange(n_instances))

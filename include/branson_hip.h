@@ -60,7 +60,8 @@ enum {
 typedef struct bh_options {
   int32_t precision;     /* BH_PRECISION_*                                  */
   int32_t algorithm;     /* BH_ALGO_*                                       */
-  int32_t block_size;    /* work-items per block, 0 = default (256)         */
+  int32_t block_size;    /* tracking work-items per block: 64, 128 or 256;
+                            0 = default (256)                              */
   int32_t blocks_per_cu; /* persistent kernel occupancy, 0 = default        */
   int32_t tally_mode;    /* 0 auto, 1 global atomics, 2 block-private LDS,
                             3 global atomics with the track tally summed per
@@ -73,6 +74,10 @@ typedef struct bh_options {
                                carry; a history that would exceed it is
                                finished in FP64.  Bounds every cell's tally
                                error against FP64.  0 = default (1e-6)     */
+  int32_t batch_photons;    /* persistent: photons a wave claims at a time
+                               (1..2^20), 0 = default (256)                */
+  int32_t step_graph;       /* host driver: run each step's device phase as
+                               one HIP graph (bh_step_graph), 0 = no      */
 } bh_options;
 
 /* Read-only cell data for one step (src/cell.h:319-336). */
@@ -167,6 +172,25 @@ int bh_source_from_energy(bh_handle *h, const double *emission_E,
                           double total_E, uint32_t seed, uint32_t cycle,
                           uint32_t rank, double dt, uint64_t *n_new);
 
+/* Several independent problems in one engine (no counterpart in the
+ * reference; the batched workload of many problems of uneven size).  The
+ * mesh of bh_set_mesh is the concatenation of the instances' cells:
+ * instance k owns cells [first_cell[k], first_cell[k+1]) with
+ * first_cell[0] = 0 and first_cell[n_instances] = n_cells, and neighbour
+ * indices are global.  Each instance draws from its own seed; source,
+ * transport and census of all instances then run in the same launches, so
+ * short and long problems share the GPU.  Clears the resident census;
+ * n_instances = 0 returns to a single problem.  Stays in force across
+ * bh_set_mesh calls with the same number of cells. */
+int bh_set_instances(bh_handle *h, uint32_t n_instances, const uint32_t *first_cell,
+                     const uint32_t *seeds);
+/* bh_source for every instance at once: instance k samples with
+ * n_user_photons[k] and total_E[k], streams numbered within the instance
+ * exactly as bh_source numbers them for a lone problem.  The per-cell
+ * energies come from bh_set_source_energy. */
+int bh_source_instances(bh_handle *h, const uint64_t *n_user_photons, const double *total_E,
+                        uint32_t cycle, uint32_t rank, double dt, uint64_t *n_new);
+
 /* Transports every resident photon (new + census), tallies per-cell absorbed
  * and track-length energy, and rebuilds the census in transport order with
  * life_dx = c * next_dt.  Replaces replicated_transport
@@ -174,6 +198,20 @@ int bh_source_from_energy(bh_handle *h, const double *emission_E,
  * overwritten when non-NULL; stats may be NULL. */
 int bh_transport(bh_handle *h, double next_dt, double *abs_E, double *track_E,
                  bh_step_stats *stats);
+
+/* One step's device phase -- bh_source followed by bh_transport, same
+ * arguments and results -- as a single HIP graph launch (no counterpart in
+ * the reference; the north star's "each time step captured as a graph").
+ * Every photon count stays on the device: the host computes the step's
+ * photon count exactly (the source's own arithmetic) and the graph's first
+ * node copies the step's values in, so no host round trip splits the step.
+ * The graph is captured on first use and replayed; it is captured again
+ * when a buffer is reallocated or the mesh changes.  Persistent algorithm,
+ * one problem; the first step (initial census) runs as bh_source +
+ * bh_transport.  stats->transport_ms is the whole graph's device time. */
+int bh_step_graph(bh_handle *h, uint64_t n_user_photons, double total_E, uint32_t seed,
+                  uint32_t cycle, uint32_t rank, double dt, double next_dt, double *abs_E,
+                  double *track_E, bh_step_stats *stats);
 
 /* Copies the last step's tallies out (n_cells doubles each; either may be
  * NULL).  Lets a caller keep the copy out of a timed transport call. */

@@ -96,9 +96,10 @@ def test_adjoint_matches_central_differences_on_the_field_problem():
     n = prob.n_cells
     sa, f = prob.sigma_a, prob.fleck
     z = problems.lognormal_field((32, 32, 32), 1.0, 1.0, 4.0, 77)  # exp(z), smooth
-    target_sa = sa * np.power(z, 0.1)
-    va = sa * np.log(problems.lognormal_field((32, 32, 32), 1.0, 1.0, 4.0, 5))
-    vf = 0.2 * f * np.log(problems.lognormal_field((32, 32, 32), 1.0, 1.0, 4.0, 6))
+    target_sa = sa * 1.05 * np.power(z, 0.1)  # mostly more absorbing: g one-signed
+    # one-signed smooth directions, so the derivative is resolved
+    va = sa * problems.lognormal_field((32, 32, 32), 1.0, 0.5, 4.0, 5)
+    vf = -0.2 * f * problems.lognormal_field((32, 32, 32), 1.0, 0.5, 4.0, 6)
     h = 1e-4
     with capi.Engine(0, algorithm=capi.ALGO_PERSISTENT) as e:
         src = FixedSource.sample(e, prob, 10**6, seed=2718)
@@ -134,7 +135,9 @@ def test_autograd_backward_is_the_adjoint():
         out = absorbed_energy(sa, f, src)
         loss = ((out - target) ** 2).sum()
         loss.backward()
-        assert np.array_equal(out.detach().numpy(), src.absorbed(prob.sigma_a, prob.fleck))
+        # the forward pass is the transport (tally atomics: summation order varies)
+        assert np.allclose(out.detach().numpy(), src.absorbed(prob.sigma_a, prob.fleck),
+                           rtol=1e-12, atol=0)
         g = 2.0 * (out.detach().numpy() - target.numpy())
         da, df = src.gradient(prob.sigma_a, prob.fleck, g)
     assert np.allclose(sa.grad.numpy(), da, rtol=1e-12, atol=0)

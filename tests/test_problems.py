@@ -62,3 +62,17 @@ def test_with_params_keeps_the_source():
     assert np.array_equal(q.emission_E, p.emission_E)
     assert np.array_equal(q.sigma_a, 2.0 * p.sigma_a)
     assert np.array_equal(q.fleck, p.fleck)
+
+
+def test_batch_concatenation_keeps_instances_apart():
+    b = problems.batched_instances(n_instances=10, seed=1, cells_range=(20, 400),
+                                   photons_range=(100, 1000))
+    inst = b.instance_of_cell()
+    assert len(inst) == b.n_cells == b.first_cell[-1]
+    inner = b.bc == capi.ELEMENT
+    rows, faces = np.nonzero(inner)
+    # every neighbour lies in the same instance
+    assert np.array_equal(inst[b.next[rows, faces]], inst[rows])
+    assert np.array_equal(b.seeds, np.arange(10))
+    sizes = np.diff(b.first_cell)
+    assert np.all(sizes > 0)
