@@ -55,7 +55,7 @@ class Options(C.Structure):
                 ("blocks_per_cu", C.c_int32), ("tally_mode", C.c_int32),
                 ("refill_threshold", C.c_int32), ("reserve_photons", C.c_uint64),
                 ("fp32_tol", C.c_double), ("batch_photons", C.c_int32),
-                ("step_graph", C.c_int32)]
+                ("step_graph", C.c_int32), ("general_mesh", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Cells(C.Structure):
@@ -70,7 +70,7 @@ class StepStats(C.Structure):
                 ("n_exit", C.c_uint64), ("n_killed", C.c_uint64), ("n_segments", C.c_uint64),
                 ("n_flushes", C.c_uint64), ("transport_ms", C.c_double),
                 ("census_ms", C.c_double), ("source_ms", C.c_double),
-                ("n_replayed", C.c_uint64)]
+                ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("pad", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -130,11 +130,12 @@ class Engine:
     def __init__(self, device: int = 0, precision: int = PRECISION_FP64,
                  algorithm: int = ALGO_HISTORY, tally_mode: int = 0, blocks_per_cu: int = 0,
                  refill_threshold: int = 0, reserve_photons: int = 0, fp32_tol: float = 0.0,
-                 block_size: int = 0, batch_photons: int = 0):
+                 block_size: int = 0, batch_photons: int = 0, general_mesh: bool = False):
         L = lib()
         self._h = C.c_void_p()
         self.opts = Options(precision, algorithm, block_size, blocks_per_cu, tally_mode,
-                            refill_threshold, reserve_photons, fp32_tol, batch_photons, 0)
+                            refill_threshold, reserve_photons, fp32_tol, batch_photons, 0,
+                            int(general_mesh), 0)
         rc = L.bh_create(device, C.byref(self.opts), C.byref(self._h))
         if rc != BH_OK:
             msg = L.bh_last_error(self._h).decode() if self._h else "bh_create failed"
@@ -394,12 +395,13 @@ class Driver:
                  precision: int = PRECISION_FP64, algorithm: int = ALGO_HISTORY,
                  tally_mode: int = 0, blocks_per_cu: int = 0, refill_threshold: int = 0,
                  reserve_photons: int = 0, fp32_tol: float = 0.0, block_size: int = 0,
-                 batch_photons: int = 0, step_graph: bool = False):
+                 batch_photons: int = 0, step_graph: bool = False,
+                 general_mesh: bool = False):
         L = _host_lib()
         self._d = C.c_void_p()
         cfg = DriverConfig(Options(precision, algorithm, block_size, blocks_per_cu, tally_mode,
                                    refill_threshold, reserve_photons, fp32_tol, batch_photons,
-                                   int(step_graph)), device, n_groups,
+                                   int(step_graph), int(general_mesh), 0), device, n_groups,
                            rank, n_ranks, n_photons, t_stop)
         rc = L.bh_driver_create(os.fsencode(xml_path), C.byref(cfg), C.byref(self._d))
         if rc != BH_OK:

@@ -208,6 +208,20 @@ struct alignas(64) CellGeo {
 };
 static_assert(sizeof(CellGeo) == 64, "CellGeo must be one 64-byte record");
 
+// Lattice meshes (a rectilinear grid numbered x fastest, opacities equal in
+// every group -- every mesh Branson builds from a deck): the node
+// coordinates live in three small axis arrays and a cell's per-segment data
+// in one 32-byte record, half a cache line, so the cell data of a 64^3 mesh
+// fits in one XCD's L2.
+struct alignas(32) CellRec {
+  double sigma_a;
+  double sigma_s;
+  double fleck;
+  uint32_t bc_bits;  // as CellGeo
+  uint32_t pad;
+};
+static_assert(sizeof(CellRec) == 32, "CellRec must be a 32-byte record");
+
 __device__ __forceinline__ int face_bc(uint32_t bits, uint32_t face) {
   return (int)((bits >> (3u * face)) & 7u);
 }

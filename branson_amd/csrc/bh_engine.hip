@@ -91,6 +91,10 @@ struct bh_handle {
   bool has_mesh = false;
   bool gray_sigma_a = false;     // one tally atomic per flush (see with_tally)
   bool uniform_sigma_a = false;  // sigma_a of every cell equal in all groups
+  bool lattice = false;          // mesh is a lattice: CellRec + axis arrays
+  uint32_t dims[3] = {0, 0, 0};
+  DArr<CellRec> rec;
+  DArr<double> axes;             // x nodes, then y, then z
 
   // per-cell source energies
   DArr<double> em, src, cen;
@@ -224,9 +228,18 @@ int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
 DevCells dev_cells(const bh_handle* h) {
   const double tol = h->opt.fp32_tol > 0.0 ? h->opt.fp32_tol : 1e-6;
   const bool inst = h->n_inst > 0;
-  return DevCells{h->geo.p,        h->sigma.p, h->next.p,
-                  h->n_cells,      h->G,       (float)tol,
-                  inst ? h->cell_inst.p : nullptr, inst ? h->inst_ctr_hi.p : nullptr};
+  DevCells d{h->geo.p,        h->sigma.p, h->next.p,
+             h->n_cells,      h->G,       (float)tol,
+             inst ? h->cell_inst.p : nullptr, inst ? h->inst_ctr_hi.p : nullptr,
+             nullptr,         {nullptr, nullptr, nullptr}, {0, 0, 0}};
+  if (h->lattice) {
+    d.rec = h->rec.p;
+    d.axis[0] = h->axes.p;
+    d.axis[1] = h->axes.p + h->dims[0] + 1;
+    d.axis[2] = d.axis[1] + h->dims[1] + 1;
+    for (int a = 0; a < 3; ++a) d.dims[a] = h->dims[a];
+  }
+  return d;
 }
 
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)div_up(n, block); }
@@ -259,6 +272,64 @@ int read_total(bh_handle* h, int slot, uint64_t* v) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   *v = *hp;
   return BH_OK;
+}
+
+// A lattice: cells numbered x fastest on a tensor-product grid whose node
+// coordinates agree bit for bit with three axis arrays, ELEMENT faces whose
+// neighbour is the lattice neighbour, no PROCESSOR face, and opacities equal
+// in every group of a cell.  Every mesh built from a Branson deck is one.
+bool detect_lattice(const bh_cells* c, uint32_t dims[3], std::vector<double>& axes) {
+  const uint32_t n = c->n_cells, G = c->n_groups;
+  const double* nd = c->nodes;
+  auto at = [&](uint32_t cell, int k) { return nd[(size_t)cell * 6 + k]; };
+  uint32_t nx = 1;
+  while (nx < n && at(nx, 2) == at(0, 2) && at(nx, 4) == at(0, 4) && at(nx, 0) == at(nx - 1, 1))
+    ++nx;
+  uint32_t ny = 1;
+  while ((uint64_t)(ny + 1) * nx <= n && at(ny * nx, 4) == at(0, 4) &&
+         at(ny * nx, 2) == at((ny - 1) * nx, 3))
+    ++ny;
+  const uint64_t nxy = (uint64_t)nx * ny;
+  if (n % nxy) return false;
+  const uint32_t nz = (uint32_t)(n / nxy);
+  axes.assign((size_t)nx + ny + nz + 3, 0.0);
+  double* X = axes.data();
+  double* Y = X + nx + 1;
+  double* Z = Y + ny + 1;
+  for (uint32_t i = 0; i < nx; ++i) X[i] = at(i, 0);
+  X[nx] = at(nx - 1, 1);
+  for (uint32_t j = 0; j < ny; ++j) Y[j] = at(j * nx, 2);
+  Y[ny] = at((ny - 1) * nx, 3);
+  for (uint32_t k = 0; k < nz; ++k) Z[k] = at((uint32_t)(k * nxy), 4);
+  Z[nz] = at((uint32_t)((nz - 1) * nxy), 5);
+  const uint64_t stride[3] = {1, nx, nxy};
+  const uint32_t dim[3] = {nx, ny, nz};
+  for (uint32_t k = 0, cell = 0; k < nz; ++k)
+    for (uint32_t j = 0; j < ny; ++j)
+      for (uint32_t i = 0; i < nx; ++i, ++cell) {
+        const double want[6] = {X[i], X[i + 1], Y[j], Y[j + 1], Z[k], Z[k + 1]};
+        const uint32_t idx[3] = {i, j, k};
+        for (int f = 0; f < 6; ++f) {
+          if (std::memcmp(&want[f], &nd[(size_t)cell * 6 + f], sizeof(double)) != 0) return false;
+          const int32_t bc = c->bc[(size_t)cell * 6 + f];
+          if (bc == BH_PROCESSOR) return false;
+          if (bc == BH_ELEMENT) {
+            const int a = f >> 1;
+            const bool hi = f & 1;
+            if (hi ? idx[a] + 1 >= dim[a] : idx[a] == 0) return false;
+            const uint64_t nb = hi ? cell + stride[a] : cell - stride[a];
+            if (c->next[(size_t)cell * 6 + f] != nb) return false;
+          }
+        }
+        for (uint32_t g = 1; g < G; ++g)
+          if (c->sigma_a[(size_t)cell * G + g] != c->sigma_a[(size_t)cell * G] ||
+              c->sigma_s[(size_t)cell * G + g] != c->sigma_s[(size_t)cell * G])
+            return false;
+      }
+  dims[0] = nx;
+  dims[1] = ny;
+  dims[2] = nz;
+  return true;
 }
 
 int check_cells(bh_handle* h, const bh_cells* c) {
@@ -328,6 +399,7 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n,
   const bool persistent = h->opt.algorithm == BH_ALGO_PERSISTENT;
   const bool lds = use_lds_tally(h, persistent);
   const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
+  const bool lat = h->lattice;
   ReplayList rl{nullptr, nullptr, nullptr};
   if (fp32) {
     // room for every photon of the launch, sized with the photon stores so
@@ -342,13 +414,21 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n,
     return with_tally(h, lds, [&](auto t, size_t shm) -> int {
       using T = decltype(t);
       if (fp32) {
-        k_transport_history<T, float><<<grid_for(n, block), block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+        if (lat)
+          k_transport_history<T, float, true><<<grid_for(n, block), block, shm, h->stream>>>(
+              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+        else
+          k_transport_history<T, float, false><<<grid_for(n, block), block, shm, h->stream>>>(
+              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
         HIPCHK(h, hipGetLastError());
         return launch_replay(h, cells, W, rl, t, shm);
       }
-      k_transport_history<T, double><<<grid_for(n, block), block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+      if (lat)
+        k_transport_history<T, double, true><<<grid_for(n, block), block, shm, h->stream>>>(
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+      else
+        k_transport_history<T, double, false><<<grid_for(n, block), block, shm, h->stream>>>(
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
       HIPCHK(h, hipGetLastError());
       return BH_OK;
     });
@@ -363,13 +443,21 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n,
   return with_tally(h, lds, [&](auto t, size_t shm) -> int {
     using T = decltype(t);
     if (fp32) {
-      k_transport_persistent<T, float><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+      if (lat)
+        k_transport_persistent<T, float, true><<<grid, block, shm, h->stream>>>(
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+      else
+        k_transport_persistent<T, float, false><<<grid, block, shm, h->stream>>>(
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
       HIPCHK(h, hipGetLastError());
       return launch_replay(h, cells, W, rl, t, shm);
     }
-    k_transport_persistent<T, double><<<grid, block, shm, h->stream>>>(
-        cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+    if (lat)
+      k_transport_persistent<T, double, true><<<grid, block, shm, h->stream>>>(
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+    else
+      k_transport_persistent<T, double, false><<<grid, block, shm, h->stream>>>(
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   });
@@ -420,7 +508,8 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
   }
   if (h->opt.precision < 0 || h->opt.precision > 1 || h->opt.algorithm < 0 ||
       h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 3 ||
-      !(h->opt.fp32_tol >= 0.0 && h->opt.fp32_tol < 1.0)) {
+      !(h->opt.fp32_tol >= 0.0 && h->opt.fp32_tol < 1.0) || h->opt.general_mesh < 0 ||
+      h->opt.general_mesh > 1 || h->opt.reserved != 0) {
     int r = fail(h, BH_ERR_ARG, "invalid bh_options");
     *out = h;
     return r;
@@ -474,6 +563,8 @@ void bh_destroy(bh_handle* h) {
   release(h->tiles2);
   release(h->parts);
   release(h->aos);
+  release(h->rec);
+  release(h->axes);
   release(h->cell_inst);
   release(h->inst_ctr_hi);
   release(h->inst_src);
@@ -554,6 +645,24 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
   h->n_cells = n;
   h->G = G;
   h->has_mesh = true;
+  std::vector<double> axes;
+  uint32_t dims[3];
+  h->lattice = false;
+  if (h->opt.general_mesh == 0 && detect_lattice(c, dims, axes)) {
+    std::vector<CellRec> rec(n);
+    for (uint32_t i = 0; i < n; ++i)
+      rec[i] = CellRec{c->sigma_a[(size_t)i * G], c->sigma_s[(size_t)i * G], c->fleck[i],
+                       geo[i].bc_bits, 0u};
+    TRY(ensure(h, h->rec, n));
+    TRY(ensure(h, h->axes, axes.size()));
+    HIPCHK(h, hipMemcpyAsync(h->rec.p, rec.data(), n * sizeof(CellRec), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->axes.p, axes.data(), axes.size() * sizeof(double),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int a = 0; a < 3; ++a) h->dims[a] = dims[a];
+    h->lattice = true;
+  }
   h->uniform_sigma_a = true;
   for (uint32_t i = 0; i < n && h->uniform_sigma_a; ++i)
     for (uint32_t g = 1; g < G; ++g)
@@ -795,6 +904,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
+    stats->lattice = h->lattice ? 1 : 0;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     stats->census_ms = elapsed_ms(h->ev[1], h->ev[2]);
     stats->source_ms = h->last_source_ms;
@@ -829,6 +939,7 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
           u(h->tiles.p),     u(h->tiles2.p),  u(h->parts.p),        u(h->abs_E.p),
           u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
           u(h->h_tally),     (uintptr_t)h->n_cells, (uintptr_t)h->G,
+          u(h->rec.p),       u(h->axes.p),    (uintptr_t)h->lattice,
           (uintptr_t)h->gray_sigma_a, (uintptr_t)h->ctr_hi};
 }
 
@@ -985,6 +1096,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
+    stats->lattice = h->lattice ? 1 : 0;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph
   }
   if (ctrs.stuck)
@@ -1140,6 +1252,7 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
+    stats->lattice = h->lattice ? 1 : 0;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     for (uint64_t i = 0; i < n; ++i) {
       const uint8_t d = photons[i].descriptor[0];

@@ -34,6 +34,11 @@ struct DevCells {
   // each cell and each instance's RNG counter-high word (seed << 32)
   const uint32_t* cell_inst;
   const uint64_t* inst_ctr_hi;
+  // lattice meshes (else rec is null): per-cell records, axis node arrays
+  // (nx+1, ny+1, nz+1 entries) and the lattice shape
+  const CellRec* rec;
+  const double* axis[3];
+  uint32_t dims[3];
 };
 
 // The RNG counter-high word of a photon in cell c.

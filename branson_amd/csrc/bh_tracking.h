@@ -70,6 +70,7 @@ struct Live {
   // (eL), all in cm, and the relative error of the weight (eE)
   float eP, eT, eL, eE;
   double E_handoff;    // FP64 replay: the FP32 weight when FP32 stopped
+  uint32_t ijk[3];     // lattice meshes: the cell's (i, j, k)
   uint8_t desc;
   bool pending;  // an effective scatter awaits its draws
 };
@@ -81,16 +82,78 @@ constexpr float kU = 0x1p-24f;
 constexpr float kDir = 0x1p-22f;
 constexpr float kE = 0x1p-21f;
 
-template <typename R>
+// The data one segment reads about its cell.  kLat = false: the 64-byte
+// CellGeo record and the per-group opacities; kLat = true (lattice meshes):
+// the 32-byte CellRec and the axis node arrays, indexed by the photon's
+// (i, j, k).  node(k) is the face coordinate k = 2 * axis + (high side).
+// Both views return the same numbers for the same cell.
+template <bool kLat>
+struct CellView;
+
+template <>
+struct CellView<false> {
+  const CellGeo* geo;
+  const double2* sig;
+  double f;
+  __device__ __forceinline__ CellView(const DevCells& cells, uint32_t c, const uint32_t*)
+      : geo(&cells.geo[c]), sig(&cells.sigma[(size_t)c * cells.G]), f(geo->fleck) {}
+  __device__ __forceinline__ double2 sigma(uint32_t g) const { return sig[g]; }
+  __device__ __forceinline__ double node(uint32_t k) const { return geo->nodes[k]; }
+  __device__ __forceinline__ uint32_t bc_bits() const { return geo->bc_bits; }
+};
+
+// One of three values by a run-time index, as selects: indexing an array
+// with a run-time value would put the array (and what holds it) in scratch
+// memory instead of registers.
+template <typename T>
+__device__ __forceinline__ T pick3(T a, T b, T c, uint32_t i) {
+  return i == 0 ? a : (i == 1 ? b : c);
+}
+
+template <>
+struct CellView<true> {
+  const double* ax0;
+  const double* ax1;
+  const double* ax2;
+  uint32_t i0, i1, i2;
+  double sa, ss, f;
+  uint32_t bits;
+  __device__ __forceinline__ CellView(const DevCells& c, uint32_t cell, const uint32_t* ijk)
+      : ax0(c.axis[0]), ax1(c.axis[1]), ax2(c.axis[2]), i0(ijk[0]), i1(ijk[1]), i2(ijk[2]) {
+    const CellRec r = c.rec[cell];
+    sa = r.sigma_a;
+    ss = r.sigma_s;
+    f = r.fleck;
+    bits = r.bc_bits;
+  }
+  __device__ __forceinline__ double2 sigma(uint32_t) const { return make_double2(sa, ss); }
+  __device__ __forceinline__ double node(uint32_t k) const {
+    const uint32_t a = k >> 1;
+    return pick3(ax0, ax1, ax2, a)[pick3(i0, i1, i2, a) + (k & 1u)];
+  }
+  __device__ __forceinline__ uint32_t bc_bits() const { return bits; }
+};
+
+// (i, j, k) of cell c on a lattice numbered x fastest.
+__device__ __forceinline__ void lattice_ijk(const DevCells& cells, uint32_t c, uint32_t ijk[3]) {
+  const uint32_t nxy = cells.dims[0] * cells.dims[1];
+  ijk[2] = c / nxy;
+  const uint32_t r = c - ijk[2] * nxy;
+  ijk[1] = r / cells.dims[0];
+  ijk[0] = r - ijk[1] * cells.dims[0];
+}
+
+template <typename R, bool kLat = false>
 __device__ __forceinline__ void load_live(const DevCells& cells, const PhotonSoA& W, uint64_t i,
                                           uint64_t ctr_hi, Live<R>& L) {
   L.cell = W.cell[i];
+  if constexpr (kLat) lattice_ijk(cells, L.cell, L.ijk);
   if constexpr (sizeof(R) == 4) {
-    const CellGeo* g = &cells.geo[L.cell];
+    const CellView<kLat> cv(cells, L.cell, L.ijk);
     L.frame = L.cell;
-    L.x = (float)(W.x[i] - g->nodes[0]);
-    L.y = (float)(W.y[i] - g->nodes[2]);
-    L.z = (float)(W.z[i] - g->nodes[4]);
+    L.x = (float)(W.x[i] - cv.node(0));
+    L.y = (float)(W.y[i] - cv.node(2));
+    L.z = (float)(W.z[i] - cv.node(4));
     L.life = (float)W.life[i];
     L.eP = kU * (fabsf(L.x) + fabsf(L.y) + fabsf(L.z));
     L.eT = L.eP;
@@ -120,14 +183,17 @@ __device__ __forceinline__ void load_live(const DevCells& cells, const PhotonSoA
   L.pending = false;
 }
 
-template <typename R>
+template <typename R, bool kLat = false>
 __device__ __forceinline__ void store_live(const DevCells& cells, const PhotonSoA& W, uint64_t i,
                                            const Live<R>& L) {
   if constexpr (sizeof(R) == 4) {
-    const CellGeo* g = &cells.geo[L.frame];
-    W.x[i] = g->nodes[0] + (double)L.x;
-    W.y[i] = g->nodes[2] + (double)L.y;
-    W.z[i] = g->nodes[4] + (double)L.z;
+    // lattice meshes have no PROCESSOR faces, so the frame is the cell
+    const double x0 = kLat ? cells.axis[0][L.ijk[0]] : cells.geo[L.frame].nodes[0];
+    const double y0 = kLat ? cells.axis[1][L.ijk[1]] : cells.geo[L.frame].nodes[2];
+    const double z0 = kLat ? cells.axis[2][L.ijk[2]] : cells.geo[L.frame].nodes[4];
+    W.x[i] = x0 + (double)L.x;
+    W.y[i] = y0 + (double)L.y;
+    W.z[i] = z0 + (double)L.z;
   } else {
     W.x[i] = L.x;
     W.y[i] = L.y;
@@ -163,16 +229,61 @@ __device__ __forceinline__ void flush(Flush& fo, const Live<R>& L) {
 // the crossing axis; zero shift on the other axes of a rectilinear mesh).
 // A PROCESSOR face ends the history with the neighbour's index, which may
 // not be a cell of this mesh, so there the frame stays with the old cell.
-__device__ __forceinline__ void enter_cell(const DevCells&, const CellGeo*, uint32_t nc,
-                                           Live<double>& L) {
+// The neighbour across face sc (ELEMENT or PROCESSOR); on a lattice the
+// neighbour is the next index along the face's axis.
+template <bool kLat, typename R>
+__device__ __forceinline__ uint32_t neighbour(const DevCells& cells, const Live<R>& L) {
+  if constexpr (kLat) {
+    const uint32_t a = L.sc >> 1;
+    const uint32_t stride = (a == 0) ? 1u : (a == 1) ? cells.dims[0] : cells.dims[0] * cells.dims[1];
+    return (L.sc & 1u) ? L.cell + stride : L.cell - stride;
+  } else {
+    return cells.next[(size_t)L.cell * 6 + L.sc];
+  }
+}
+
+// Lattice: (i, j, k) moves one step along the crossing face's axis.  The
+// components are updated by selects (no run-time array index).
+template <typename R>
+__device__ __forceinline__ uint32_t lattice_step(Live<R>& L, uint32_t* from) {
+  const uint32_t a = L.sc >> 1;
+  const uint32_t i0 = pick3(L.ijk[0], L.ijk[1], L.ijk[2], a);
+  const uint32_t i1 = (L.sc & 1u) ? i0 + 1u : i0 - 1u;
+  L.ijk[0] = a == 0 ? i1 : L.ijk[0];
+  L.ijk[1] = a == 1 ? i1 : L.ijk[1];
+  L.ijk[2] = a == 2 ? i1 : L.ijk[2];
+  *from = i0;
+  return i1;
+}
+
+template <bool kLat>
+__device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView<kLat>&,
+                                           uint32_t nc, Live<double>& L) {
+  if constexpr (kLat) {
+    uint32_t i0;
+    (void)lattice_step(L, &i0);
+  }
   L.cell = nc;
 }
-__device__ __forceinline__ void enter_cell(const DevCells& cells, const CellGeo* from,
+template <bool kLat>
+__device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView<kLat>& from,
                                            uint32_t nc, Live<float>& L) {
-  const CellGeo* to = &cells.geo[nc];
-  const float sx = (float)(to->nodes[0] - from->nodes[0]);
-  const float sy = (float)(to->nodes[2] - from->nodes[2]);
-  const float sz = (float)(to->nodes[4] - from->nodes[4]);
+  float sx, sy, sz;
+  if constexpr (kLat) {  // only the crossing axis moves
+    const uint32_t a = L.sc >> 1;
+    uint32_t i0;
+    const uint32_t i1 = lattice_step(L, &i0);
+    const double* ax = pick3(cells.axis[0], cells.axis[1], cells.axis[2], a);
+    const float s = (float)(ax[i1] - ax[i0]);
+    sx = (a == 0) ? s : 0.0f;
+    sy = (a == 1) ? s : 0.0f;
+    sz = (a == 2) ? s : 0.0f;
+  } else {
+    const CellGeo* to = &cells.geo[nc];
+    sx = (float)(to->nodes[0] - from.node(0));
+    sy = (float)(to->nodes[2] - from.node(2));
+    sz = (float)(to->nodes[4] - from.node(4));
+  }
   L.x -= sx;
   L.y -= sy;
   L.z -= sz;
@@ -186,27 +297,30 @@ __device__ __forceinline__ void enter_cell(const DevCells& cells, const CellGeo*
 // A reflection mirrors both the exact and the FP32 continuation about the
 // face; the two mirror planes differ by the FP32 rounding of the face
 // offset, which is all a reflection adds to the error.
-__device__ __forceinline__ void reflect_error(const CellGeo*, uint32_t, Live<double>&) {}
-__device__ __forceinline__ void reflect_error(const CellGeo* geo, uint32_t axis, Live<float>& L) {
-  const float w = (float)(geo->nodes[2 * axis + 1] - geo->nodes[2 * axis]);
+template <bool kLat>
+__device__ __forceinline__ void reflect_error(const CellView<kLat>&, uint32_t, Live<double>&) {}
+template <bool kLat>
+__device__ __forceinline__ void reflect_error(const CellView<kLat>& cv, uint32_t axis,
+                                              Live<float>& L) {
+  const float w = (float)(cv.node(2 * axis + 1) - cv.node(2 * axis));
   L.eP += 2.0f * kU * w;
   L.eT += 2.0f * kU * w;
 }
 
-template <typename R>
-__device__ __forceinline__ bool boundary_event(const DevCells& cells, const CellGeo* geo,
+template <bool kLat, typename R>
+__device__ __forceinline__ bool boundary_event(const DevCells& cells, const CellView<kLat>& cv,
                                                Live<R>& L, Flush& fo) {
-  const int bc = face_bc(geo->bc_bits, L.sc);
+  const int bc = face_bc(cv.bc_bits(), L.sc);
   if (bc == kElement) {
     flush(fo, L);
-    enter_cell(cells, geo, cells.next[(size_t)L.cell * 6 + L.sc], L);
+    enter_cell(cells, cv, neighbour<kLat>(cells, L), L);
     L.desc = kBound;
     L.t_abs = 0.0;
     L.t_track = 0.0;
     return false;
   }
-  if (bc == kProcessor) {
-    const uint32_t nc = cells.next[(size_t)L.cell * 6 + L.sc];
+  if (bc == kProcessor) {  // (never on a lattice mesh)
+    const uint32_t nc = neighbour<kLat>(cells, L);
     L.desc = kPass;
     flush(fo, L);
     L.cell = nc;
@@ -225,7 +339,7 @@ __device__ __forceinline__ bool boundary_event(const DevCells& cells, const Cell
   L.u = (axis == 0) ? -L.u : L.u;
   L.v = (axis == 1) ? -L.v : L.v;
   L.w = (axis == 2) ? -L.w : L.w;
-  reflect_error(geo, axis, L);
+  reflect_error(cv, axis, L);
   L.desc = kBound;
   return false;
 }
@@ -246,7 +360,7 @@ struct SegInfo {
 };
 
 // ---- FP64: the reference's arithmetic, operation for operation ----------
-template <bool kTrack, bool kInfo = false>
+template <bool kTrack, bool kLat = false, bool kInfo = false>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, Flush& fo,
                                         SegInfo* si = nullptr) {
   if constexpr (kInfo) {
@@ -264,9 +378,9 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
     return true;
   }
   const uint32_t G = cells.G;
-  const CellGeo* geo = &cells.geo[L.cell];
-  const double f = geo->fleck;
-  double2 sg = cells.sigma[(size_t)L.cell * G + L.group];
+  const CellView<kLat> cv(cells, L.cell, L.ijk);
+  const double f = cv.f;
+  double2 sg = cv.sigma(L.group);
   double total_sigma_s = (1.0 - f) * sg.x + sg.y;
 
   double d_s = 1.0e100;
@@ -297,20 +411,19 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
       }
       st = resample ? kDrawGroup : kDrawDistance;
     } else if (st == kDrawGroup) {  // sample_emission_group walk (:98)
-      const double2* sa = &cells.sigma[(size_t)L.cell * G];
       double cdf = r;
       int g = -1;
-      const double norm = 1.0 / (sa[0].x * (double)G);
+      const double norm = 1.0 / (cv.sigma(0).x * (double)G);
       while (cdf > 0) {
         g++;
         if ((uint32_t)g >= G) {
           g = (int)G - 1;
           break;
         }
-        cdf -= sa[g].x * norm;
+        cdf -= cv.sigma((uint32_t)g).x * norm;
       }
       L.group = (uint32_t)g;
-      sg = sa[g];
+      sg = cv.sigma((uint32_t)g);
       total_sigma_s = (1.0 - f) * sg.x + sg.y;
       st = (total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
     } else {  // distance to scatter (:62-63)
@@ -326,9 +439,9 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
   double d_b = 1.0e16;
   {
     const bool px = 0.0 < L.u, py = 0.0 < L.v, pz = 0.0 < L.w;
-    const double nx = px ? geo->nodes[1] : geo->nodes[0];
-    const double ny = py ? geo->nodes[3] : geo->nodes[2];
-    const double nz = pz ? geo->nodes[5] : geo->nodes[4];
+    const double nx = cv.node(px ? 1u : 0u);
+    const double ny = cv.node(py ? 3u : 2u);
+    const double nz = cv.node(pz ? 5u : 4u);
     const double dx = (nx - L.x) / L.u;
     if (dx < d_b) { d_b = dx; L.sc = px ? 1u : 0u; }
     const double dy = (ny - L.y) / L.v;
@@ -376,7 +489,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
     L.desc = kScatter;
     return false;
   }
-  if (d == d_b) return boundary_event(cells, geo, L, fo);
+  if (d == d_b) return boundary_event<kLat>(cells, cv, L, fo);
   if (d == d_c) {  // census (:133-138)
     L.desc = kCensus;
     flush(fo, L);
@@ -406,7 +519,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
 // could be off by more than fp32_tol (relative) is treated the same way, so
 // every FP32 deposit -- and with it every cell's tally -- is within fp32_tol
 // of FP64.
-template <bool kTrack>
+template <bool kTrack, bool kLat = false>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, Flush& fo) {
   if (++L.segs > kMaxSegments) {
     flush(fo, L);
@@ -414,9 +527,9 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
     return true;
   }
   const uint32_t G = cells.G;
-  const CellGeo* geo = &cells.geo[L.cell];
-  const double f = geo->fleck;
-  double2 sg = cells.sigma[(size_t)L.cell * G + L.group];
+  const CellView<kLat> cv(cells, L.cell, L.ijk);
+  const double f = cv.f;
+  double2 sg = cv.sigma(L.group);
   double total_sigma_s = (1.0 - f) * sg.x + sg.y;
 
   float d_s = 1.0e30f, e_s = 0.0f;
@@ -454,20 +567,19 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
     } else if (st == kDrawTest) {
       st = (r > (sg.y / total_sigma_s)) ? kDrawGroup : kDrawDistance;
     } else if (st == kDrawGroup) {
-      const double2* sa = &cells.sigma[(size_t)L.cell * G];
       double cdf = r;
       int g = -1;
-      const double norm = 1.0 / (sa[0].x * (double)G);
+      const double norm = 1.0 / (cv.sigma(0).x * (double)G);
       while (cdf > 0) {
         g++;
         if ((uint32_t)g >= G) {
           g = (int)G - 1;
           break;
         }
-        cdf -= sa[g].x * norm;
+        cdf -= cv.sigma((uint32_t)g).x * norm;
       }
       L.group = (uint32_t)g;
-      sg = sa[g];
+      sg = cv.sigma((uint32_t)g);
       total_sigma_s = (1.0 - f) * sg.x + sg.y;
       st = (total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
     } else {
@@ -488,9 +600,9 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
   float d_b = 1.0e16f, e_b = 0.0f, d_b2 = 1.0e16f, e_b2 = 0.0f;
   {
     const bool px = 0.0f < L.u, py = 0.0f < L.v, pz = 0.0f < L.w;
-    const float wx = (float)(geo->nodes[1] - geo->nodes[0]);
-    const float wy = (float)(geo->nodes[3] - geo->nodes[2]);
-    const float wz = (float)(geo->nodes[5] - geo->nodes[4]);
+    const float wx = (float)(cv.node(1) - cv.node(0));
+    const float wy = (float)(cv.node(3) - cv.node(2));
+    const float wz = (float)(cv.node(5) - cv.node(4));
     const float ix = 1.0f / L.u, iy = 1.0f / L.v, iz = 1.0f / L.w;
     const float dx = ((px ? wx : 0.0f) - L.x) * ix;
     const float dy = ((py ? wy : 0.0f) - L.y) * iy;
@@ -574,7 +686,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
     L.desc = kScatter;
     return false;
   }
-  if (is_b) return boundary_event(cells, geo, L, fo);
+  if (is_b) return boundary_event<kLat>(cells, cv, L, fo);
   L.desc = kCensus;
   flush(fo, L);
   return true;
@@ -637,7 +749,7 @@ __device__ __forceinline__ void replay_append(bool want, uint64_t idx, uint32_t 
 // End of a history inside a tracking kernel: store the photon, or -- FP32
 // form, ambiguous decision -- leave its initial state in place for the
 // replay.  Returns the segments this kernel applied.
-template <typename R>
+template <typename R, bool kLat>
 __device__ __forceinline__ uint32_t end_history(const DevCells& cells, const PhotonSoA& W,
                                                 uint64_t i, const Live<R>& L, bool& replay) {
   if constexpr (sizeof(R) == 4) {
@@ -646,11 +758,11 @@ __device__ __forceinline__ uint32_t end_history(const DevCells& cells, const Pho
       return L.segs - 1;
     }
   }
-  store_live(cells, W, i, L);
+  store_live<R, kLat>(cells, W, i, L);
   return L.segs;
 }
 
-template <class Tally, typename R>
+template <class Tally, typename R, bool kLat>
 BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n,
                                          uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
                                          ReplayList rl) {
@@ -660,15 +772,15 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
   bool active = i < n;
   unsigned long long flushes = 0, segs = 0, stuck = 0;
   Live<R> L;
-  if (active) load_live(cells, W, i, ctr_hi, L);
+  if (active) load_live<R, kLat>(cells, W, i, ctr_hi, L);
   while (__ballot(active) != 0) {
     Flush fo;
     fo.has = false;
     fo.kill = 0.0;
     bool replay = false;
-    if (active && segment<Tally::kTrack>(cells, L, fo)) {
+    if (active && segment<Tally::kTrack, kLat>(cells, L, fo)) {
       stuck = (L.desc == kStuck);
-      segs = end_history(cells, W, i, L, replay);
+      segs = end_history<R, kLat>(cells, W, i, L, replay);
       active = false;
     }
     if constexpr (sizeof(R) == 4) replay_append(replay, i, L.first_seg, L.E_handoff, rl, ctrs);
@@ -684,7 +796,7 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
 // claimed with one atomic on the global queue head, so the queue sees one
 // atomic per chunk histories instead of one per refill.  The loop ends
 // when the queue is drained and no lane holds a photon, so every wave exits.
-template <class Tally, typename R>
+template <class Tally, typename R, bool kLat>
 BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_t n,
                                             uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
                                             PersistentQueue* q, int refill, uint32_t chunk,
@@ -722,7 +834,7 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
         const bool take = !active && rank < avail;
         if (take) {
           idx = chunk_next + rank;
-          load_live(cells, W, idx, ctr_hi, L);
+          load_live<R, kLat>(cells, W, idx, ctr_hi, L);
           active = true;
         }
         const uint64_t taken = (avail < (uint64_t)n_idle) ? avail : (uint64_t)n_idle;
@@ -736,9 +848,9 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
     fo.has = false;
     fo.kill = 0.0;
     bool replay = false;
-    if (active && segment<Tally::kTrack>(cells, L, fo)) {
+    if (active && segment<Tally::kTrack, kLat>(cells, L, fo)) {
       stuck += (L.desc == kStuck);
-      segs += end_history(cells, W, idx, L, replay);
+      segs += end_history<R, kLat>(cells, W, idx, L, replay);
       active = false;
     }
     if constexpr (sizeof(R) == 4) replay_append(replay, idx, L.first_seg, L.E_handoff, rl, ctrs);
@@ -857,7 +969,7 @@ BH_TRACK_KERNEL void k_adjoint(DevCells cells, PhotonSoA W, uint64_t n, uint64_t
   while (active) {
     Flush fo;
     fo.has = false;
-    const bool end = segment<false, true>(cells, L, fo, &si);
+    const bool end = segment<false, false, true>(cells, L, fo, &si);
     R += g[si.c] * (si.absorbed + si.kill_E);
     if (end) {
       stuck = (L.desc == kStuck);
@@ -875,7 +987,7 @@ BH_TRACK_KERNEL void k_adjoint(DevCells cells, PhotonSoA W, uint64_t n, uint64_t
     if (active) {
       Flush fo;
       fo.has = false;
-      const bool end = segment<false, true>(cells, L, fo, &si);
+      const bool end = segment<false, false, true>(cells, L, fo, &si);
       const double gc = g[si.c];
       if (si.reemit) {  // decided before this segment's flight
         const double Rr = R - A;

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BH_ABI_VERSION 2
+#define BH_ABI_VERSION 3
 
 /* return codes */
 #define BH_OK 0
@@ -78,6 +78,12 @@ typedef struct bh_options {
                                (1..2^20), 0 = default (256)                */
   int32_t step_graph;       /* host driver: run each step's device phase as
                                one HIP graph (bh_step_graph), 0 = no      */
+  int32_t general_mesh;     /* 0: a mesh that is a lattice (rectilinear,
+                               numbered x fastest, opacities equal in every
+                               group -- any deck-built mesh) is tracked
+                               from compact per-cell records and axis
+                               arrays; 1: always the general cell records */
+  int32_t reserved;         /* set to 0                                    */
 } bh_options;
 
 /* Read-only cell data for one step (src/cell.h:319-336). */
@@ -109,6 +115,9 @@ typedef struct bh_step_stats {
   uint64_t n_replayed;    /* FP32 tracking: histories whose next decision
                              the FP32 error bound could not separate,
                              finished in FP64 from that segment on        */
+  int32_t lattice;        /* 1: tracked on the lattice records (see
+                             bh_options.general_mesh), 0: general records */
+  int32_t pad;
 } bh_step_stats;
 
 /* A photon in the reference's 120-byte AOS layout (src/photon.h:172-181). */
