@@ -72,7 +72,7 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
         with capi.Driver(deck, t_stop=DT * warmup, **kw) as w:
             w.run()
     out = dict(hist=0, segs=0, flushes=0, dev_s=0.0, step_s=0.0, k_ms=[], launches=[],
-               step1=None, replayed=0)
+               step1=None, replayed=0, lattice=None)
     with capi.Driver(deck, t_stop=DT * steps, reserve_photons=reserve, **kw) as d:
         eng = capi.lib()
         h = eng.bh_driver_engine(d._d)
@@ -90,6 +90,7 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
                 out["step1"] = d.tallies()[0]
             out["hist"] += r["transported"]
             out["replayed"] += r["n_replayed"]
+            out["lattice"] = bool(r["lattice"])
             out["segs"] += r["n_segments"]
             out["flushes"] += r["n_flushes"]
             out["dev_s"] += t1 - t0
@@ -355,6 +356,7 @@ def main(argv=None):
                                "sigma_s=1, dt=0.001 sh, 1e7 photons/step, seed 1234",
                    "form": "plain fp64 (reference RNG streams and arithmetic), "
                            "persistent waves",
+                   "cell_records": "lattice" if plain["lattice"] else "general",
                    "histories": plain["hist"], "segments": plain["segs"]},
         "segments_per_s": plain["segs"] / plain["dev_s"],
         # whole step: host physics, uploads of cells and source energies over
@@ -363,7 +365,8 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": _traffic_from_profile(),
-                     "kernel": "k_transport_persistent<GlobalTally<false>, double>",
+                     "kernel": "k_transport_persistent<GlobalTally<false>, double, %s>"
+                               % ("true" if plain["lattice"] else "false"),
                      "kernel_ms_mean": kms, "algorithmic_bytes_per_launch": bytes_launch,
                      "launches": len(plain["k_ms"]),
                      "launch_histories_segments_flushes": plain["launches"],

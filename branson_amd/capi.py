@@ -346,7 +346,7 @@ class DriverReport(C.Structure):
                 ("source_ms", C.c_double), ("transport_ms", C.c_double),
                 ("census_ms", C.c_double), ("prepare_ms", C.c_double),
                 ("device_phase_ms", C.c_double), ("finish_ms", C.c_double),
-                ("n_replayed", C.c_uint64)]
+                ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("pad", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -358,6 +358,14 @@ int with_tally(bh_handle* h, bool lds, Launch&& launch) {
     LdsTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
     return launch(t, (size_t)2 * h->n_cells * sizeof(double));
   }
+  if (h->gray_sigma_a && h->opt.tally_mode == 4 && h->n_cells < HashTally::kKill - 1u) {
+    HashTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0};
+    TRY(launch(t, (size_t)0));
+    k_tally_finalize<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(
+        h->geo.p, h->sigma.p, h->G, h->n_cells, h->abs_E.p, h->track_E.p);
+    HIPCHK(h, hipGetLastError());
+    return BH_OK;
+  }
   if (h->gray_sigma_a) {
     GlobalTally<false> t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
     TRY(launch(t, (size_t)0));
@@ -372,7 +380,7 @@ int with_tally(bh_handle* h, bool lds, Launch&& launch) {
 
 bool use_lds_tally(const bh_handle* h, bool persistent) {
   const size_t bytes = (size_t)2 * h->n_cells * sizeof(double);
-  if (h->opt.tally_mode == 1 || h->opt.tally_mode == 3) return false;
+  if (h->opt.tally_mode == 1 || h->opt.tally_mode == 3 || h->opt.tally_mode == 4) return false;
   if (h->opt.tally_mode == 2) return bytes <= h->lds_budget;
   return bytes <= (persistent ? h->lds_budget : std::min<size_t>(h->lds_budget, 16 * 1024));
 }
@@ -507,7 +515,7 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
     return r;
   }
   if (h->opt.precision < 0 || h->opt.precision > 1 || h->opt.algorithm < 0 ||
-      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 3 ||
+      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 4 ||
       !(h->opt.fp32_tol >= 0.0 && h->opt.fp32_tol < 1.0) || h->opt.general_mesh < 0 ||
       h->opt.general_mesh > 1 || h->opt.reserved != 0) {
     int r = fail(h, BH_ERR_ARG, "invalid bh_options");

@@ -347,6 +347,77 @@ __device__ __forceinline__ bool boundary_event(const DevCells& cells, const Cell
 // Draw states of a segment.
 enum : int { kDrawMu = 0, kDrawPhi, kDrawTest, kDrawGroup, kDrawDistance, kDrawDone };
 
+// ---- Wave-cooperative draws ------------------------------------------------
+// A segment takes one draw (the distance), or five after an effective
+// scatter (mu, phi, the re-emission test, the group, the distance; the
+// group draw only when the test re-samples, so the fifth may go unused).
+// Drawn lane by lane, a wave runs the Threefry block up to five times per
+// segment with most lanes idle after the first.  Instead, before the
+// segments run, the wave computes every draw its lanes may need: draw k of
+// lane o is Threefry(counter_o + k), the same value the lane's own k-th
+// call would return, and the D <= 5 * 64 draws are spread over all 64 lanes
+// in ceil(D / 64) passes.  Results go to a per-wave LDS block; the segment
+// reads them in order and advances its counter by the draws it uses.
+constexpr uint32_t kMaxDraws = 5;
+
+struct DrawLds {
+  uint16_t job[kWave * kMaxDraws];  // (owner lane << 3) | k
+  double res[kWave * kMaxDraws];
+};
+
+__device__ __forceinline__ DrawLds* draw_lds() {
+  __shared__ DrawLds d[256 / kWave];
+  return &d[threadIdx.x / kWave];
+}
+
+struct WaveDraws {
+  const double* res;
+  uint32_t next;  // index in res of the lane's next draw
+  __device__ __forceinline__ double take(Rng& rng) {
+    rng.ctr += 1u;
+    return res[next++];
+  }
+};
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, kWave);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Every lane of the wave at the same program point (full execution mask:
+// the owners' counters are read with cross-lane permutes).  `want` lanes
+// are about to run a segment.
+template <typename R>
+__device__ __forceinline__ void wave_draws(bool want, const Live<R>& L, WaveDraws& dr) {
+  const uint32_t K = want ? (L.pending ? kMaxDraws : 1u) : 0u;
+  const uint64_t m5 = __ballot(K == kMaxDraws), m1 = __ballot(K == 1u);
+  const uint32_t off = kMaxDraws * lane_rank(m5) + lane_rank(m1);
+  const uint32_t D = kMaxDraws * (uint32_t)__popcll(m5) + (uint32_t)__popcll(m1);
+  DrawLds* s = draw_lds();
+  dr.res = s->res;
+  dr.next = off;
+  if (D == 0) return;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint32_t k = 0; k < kMaxDraws; ++k)
+    if (k < K) s->job[off + k] = (uint16_t)((lane << 3) | k);
+  wave_lds_sync();
+  for (uint32_t base = 0; base < D; base += kWave) {
+    const uint32_t j = base + lane;
+    const uint32_t d = (j < D) ? (uint32_t)s->job[j] : (lane << 3);
+    const int o = (int)(d >> 3);
+    const uint64_t ctr = shfl_u64(L.rng.ctr, o) + (d & 7u);
+    const uint64_t stream = shfl_u64(L.rng.stream, o);
+    const uint64_t hi = shfl_u64(L.rng.ctr_hi, o);
+    if (j < D) {
+      uint64_t o0, o1;
+      Threefry::block(ctr, hi, stream, 0ull, o0, o1);
+      s->res[j] = u01(o0);
+    }
+  }
+  wave_lds_sync();
+}
+
 // What one FP64 segment tells the adjoint (filled when kInfo is set).
 struct SegInfo {
   uint32_t c;            // cell of the segment
@@ -362,7 +433,7 @@ struct SegInfo {
 // ---- FP64: the reference's arithmetic, operation for operation ----------
 template <bool kTrack, bool kLat = false, bool kInfo = false>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, Flush& fo,
-                                        SegInfo* si = nullptr) {
+                                        SegInfo* si = nullptr, WaveDraws* dr = nullptr) {
   if constexpr (kInfo) {
     si->reemit = 0;
     si->flight = false;
@@ -388,7 +459,8 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
   if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
   double mu = 0.0;
   while (st != kDrawDone) {
-    const double r = L.rng.next();  // the segment's only Threefry site
+    // drawn by the wave beforehand, or here (the segment's only Threefry site)
+    const double r = dr ? dr->take(L.rng) : L.rng.next();
     if (st == kDrawMu) {  // get_uniform_angle (src/sampling_functions.h:56-71)
       mu = r * 2.0 - 1.0;
       st = kDrawPhi;
@@ -520,7 +592,8 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
 // every FP32 deposit -- and with it every cell's tally -- is within fp32_tol
 // of FP64.
 template <bool kTrack, bool kLat = false>
-__device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, Flush& fo) {
+__device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, Flush& fo,
+                                        SegInfo* = nullptr, WaveDraws* dr = nullptr) {
   if (++L.segs > kMaxSegments) {
     flush(fo, L);
     L.desc = kStuck;
@@ -537,7 +610,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
   if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
   double mu = 0.0;
   while (st != kDrawDone) {
-    const double r = L.rng.next();
+    const double r = dr ? dr->take(L.rng) : L.rng.next();
     if (st == kDrawMu) {
       mu = r * 2.0 - 1.0;
       st = kDrawPhi;
@@ -772,13 +845,17 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
   bool active = i < n;
   unsigned long long flushes = 0, segs = 0, stuck = 0;
   Live<R> L;
+  L.rng = Rng{0, 0, 0};
+  L.pending = false;
   if (active) load_live<R, kLat>(cells, W, i, ctr_hi, L);
   while (__ballot(active) != 0) {
     Flush fo;
     fo.has = false;
     fo.kill = 0.0;
     bool replay = false;
-    if (active && segment<Tally::kTrack, kLat>(cells, L, fo)) {
+    WaveDraws dr;
+    wave_draws(active, L, dr);
+    if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr)) {
       stuck = (L.desc == kStuck);
       segs = end_history<R, kLat>(cells, W, i, L, replay);
       active = false;
@@ -807,6 +884,8 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
   tally.init(cells.n_cells);
   const int lane = threadIdx.x & (kWave - 1);
   Live<R> L;
+  L.rng = Rng{0, 0, 0};
+  L.pending = false;
   bool active = false;
   uint64_t idx = 0;
   uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
@@ -848,7 +927,9 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
     fo.has = false;
     fo.kill = 0.0;
     bool replay = false;
-    if (active && segment<Tally::kTrack, kLat>(cells, L, fo)) {
+    WaveDraws dr;
+    wave_draws(active, L, dr);
+    if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr)) {
       stuck += (L.desc == kStuck);
       segs += end_history<R, kLat>(cells, W, idx, L, replay);
       active = false;

@@ -65,7 +65,10 @@ typedef struct bh_options {
   int32_t blocks_per_cu; /* persistent kernel occupancy, 0 = default        */
   int32_t tally_mode;    /* 0 auto, 1 global atomics, 2 block-private LDS,
                             3 global atomics with the track tally summed per
-                            segment even for group-independent opacities   */
+                            segment even for group-independent opacities,
+                            4 global atomics of deposits combined per wave
+                            in an LDS table (group-independent opacities;
+                            otherwise as 1)                               */
   int32_t refill_threshold; /* persistent: idle lanes of a wave that trigger
                                a refill (1..64), 0 = default (16)          */
   uint64_t reserve_photons; /* initial photon capacity hint, 0 = none      */

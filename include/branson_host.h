@@ -46,6 +46,8 @@ typedef struct bh_driver_report {
   double source_ms, transport_ms, census_ms; /* device (HIP events)          */
   double prepare_ms, device_phase_ms, finish_ms; /* host wall clock per phase */
   uint64_t n_replayed;   /* FP32 tracking: histories finished in FP64       */
+  int32_t lattice;       /* 1: tracked on the lattice records               */
+  int32_t pad;
 } bh_driver_report;
 
 typedef struct bh_driver bh_driver;

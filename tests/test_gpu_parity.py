@@ -131,11 +131,13 @@ def _transport_both(d, n_user, engine_kw, n_threads=8):
 
 
 # tally_mode: 0 auto (LDS for the 100-cell slab, one global atomic per flush
-# for the cube), 1 global, 2 LDS where it fits, 3 two global atomics per flush
+# for the cube), 1 global, 2 LDS where it fits, 3 two global atomics per flush,
+# 4 deposits combined per wave in an LDS hash table
 @pytest.mark.parametrize("algo,tally", [(capi.ALGO_HISTORY, 0), (capi.ALGO_HISTORY, 1),
-                                        (capi.ALGO_HISTORY, 3), (capi.ALGO_PERSISTENT, 0),
-                                        (capi.ALGO_PERSISTENT, 1), (capi.ALGO_PERSISTENT, 2),
-                                        (capi.ALGO_PERSISTENT, 3)])
+                                        (capi.ALGO_HISTORY, 3), (capi.ALGO_HISTORY, 4),
+                                        (capi.ALGO_PERSISTENT, 0), (capi.ALGO_PERSISTENT, 1),
+                                        (capi.ALGO_PERSISTENT, 2), (capi.ALGO_PERSISTENT, 3),
+                                        (capi.ALGO_PERSISTENT, 4)])
 @pytest.mark.parametrize("deck,n_user", [("cube_40.xml", 300_000),
                                          ("marshak_slab_100.xml", 50_000)])
 def test_transport_photons_fp64_matches_oracle(deck, n_user, algo, tally):
