@@ -72,7 +72,7 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
         with capi.Driver(deck, t_stop=DT * warmup, **kw) as w:
             w.run()
     out = dict(hist=0, segs=0, flushes=0, dev_s=0.0, step_s=0.0, k_ms=[], launches=[],
-               step1=None, replayed=0, lattice=None)
+               step1=None, replayed=0, lattice=None, tally=None)
     with capi.Driver(deck, t_stop=DT * steps, reserve_photons=reserve, **kw) as d:
         eng = capi.lib()
         h = eng.bh_driver_engine(d._d)
@@ -91,6 +91,7 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
             out["hist"] += r["transported"]
             out["replayed"] += r["n_replayed"]
             out["lattice"] = bool(r["lattice"])
+            out["tally"] = TALLY_FORMS.get(r["tally"], r["tally"])
             out["segs"] += r["n_segments"]
             out["flushes"] += r["n_flushes"]
             out["dev_s"] += t1 - t0
@@ -162,6 +163,8 @@ def _cpu_baseline(steps):
 
 
 HOHLRAUM = os.path.join(ROOT, "branson_amd", "decks", "hohlraum_64.xml")
+TALLY_FORMS = {1: "one global atomic per flush (wave ring)", 2: "block-private LDS",
+               3: "two global atomics per flush", 4: "per-wave LDS hash table"}
 
 
 def _common(metric, value, steps, warmup, ms, dtype, config, **extra):
@@ -195,6 +198,8 @@ def _bench_hohlraum(a, capi):
                                 "rho=19, fill sigma_a=0.1 sigma_s=0.01 rho=0.001, 0.3 keV "
                                 "source on z=0, 30 groups, 5e7 photons/step",
                     "form": "plain fp64, persistent waves",
+                    "cell_records": "lattice" if plain["lattice"] else "general",
+                    "tally": plain["tally"],
                     "histories": plain["hist"], "segments": plain["segs"]},
                    segments_per_s=plain["segs"] / plain["dev_s"], chosen=chosen)
 
@@ -357,6 +362,7 @@ def main(argv=None):
                    "form": "plain fp64 (reference RNG streams and arithmetic), "
                            "persistent waves",
                    "cell_records": "lattice" if plain["lattice"] else "general",
+                   "tally": plain["tally"],
                    "histories": plain["hist"], "segments": plain["segs"]},
         "segments_per_s": plain["segs"] / plain["dev_s"],
         # whole step: host physics, uploads of cells and source energies over

@@ -25,7 +25,7 @@ CANDIDATES = {
     "block_size": [128, 256],
     "refill_threshold": [4, 8, 16, 32],
     "batch_photons": [64, 256, 1024],
-    "tally_mode": [0, 3],
+    "tally_mode": [0, 1, 4],
 }
 DEFAULTS = {"blocks_per_cu": 8, "block_size": 256, "refill_threshold": 16,
             "batch_photons": 256, "tally_mode": 0}

@@ -92,6 +92,8 @@ struct bh_handle {
   bool gray_sigma_a = false;     // one tally atomic per flush (see with_tally)
   bool uniform_sigma_a = false;  // sigma_a of every cell equal in all groups
   bool lattice = false;          // mesh is a lattice: CellRec + axis arrays
+  double src_neff_frac = 1.0;    // effective source cells / n_cells (tally choice)
+  int last_tally = 0;            // tally form of the last launch (bh_step_stats.tally)
   uint32_t dims[3] = {0, 0, 0};
   DArr<CellRec> rec;
   DArr<double> axes;             // x nodes, then y, then z
@@ -348,17 +350,33 @@ int check_cells(bh_handle* h, const bh_cells* c) {
   return BH_OK;
 }
 
+// The wave hash table pays where flushes concentrate on few cells (many
+// photons per cell of the source distribution: a face source, a hot spot),
+// and costs ~8% where they spread (uniform emission).  Auto mode decides by
+// the effective number of source cells (sum e)^2 / sum e^2 of the per-cell
+// source energies, as a fraction of the mesh.
+constexpr double kHashTallyNeff = 0.25;
+
+bool use_hash_tally(const bh_handle* h) {
+  if (h->opt.tally_mode == 4) return true;
+  if (h->opt.tally_mode != 0 || h->n_cells >= 0xFFFFFFFFu) return false;
+  return h->src_neff_frac < kHashTallyNeff;
+}
+
 // Runs `launch` with the tally sink chosen for this mesh: block-private LDS
-// tallies when both arrays fit, else wave-buffered global atomics -- one per
-// flush when every cell's absorption opacity is group-independent (then
-// k_tally_finalize completes the tallies), two otherwise.
+// tallies when both arrays fit, else global atomics -- one per flush when
+// every cell's absorption opacity is group-independent (then
+// k_tally_finalize completes the tallies; combined per wave in an LDS hash
+// table where the source is concentrated), two otherwise.
 template <class Launch>
 int with_tally(bh_handle* h, bool lds, Launch&& launch) {
   if (lds) {
+    h->last_tally = 2;
     LdsTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
     return launch(t, (size_t)2 * h->n_cells * sizeof(double));
   }
-  if (h->gray_sigma_a && h->opt.tally_mode == 4 && h->n_cells < HashTally::kKill - 1u) {
+  if (h->gray_sigma_a && use_hash_tally(h)) {
+    h->last_tally = 4;
     HashTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0};
     TRY(launch(t, (size_t)0));
     k_tally_finalize<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(
@@ -367,6 +385,7 @@ int with_tally(bh_handle* h, bool lds, Launch&& launch) {
     return BH_OK;
   }
   if (h->gray_sigma_a) {
+    h->last_tally = 1;
     GlobalTally<false> t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
     TRY(launch(t, (size_t)0));
     k_tally_finalize<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(
@@ -374,6 +393,7 @@ int with_tally(bh_handle* h, bool lds, Launch&& launch) {
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   }
+  h->last_tally = 3;
   GlobalTally<true> t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
   return launch(t, (size_t)0);
 }
@@ -709,6 +729,15 @@ int bh_set_source_energy(bh_handle* h, const double* emission_E, const double* s
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->em_host.assign(emission_E, emission_E + n);
   h->src_host.assign(source_E, source_E + n);
+  {  // photons are drawn in proportion to energy: (sum e)^2 / sum e^2 cells
+    double s1 = 0.0, s2 = 0.0;
+    for (uint32_t c = 0; c < n; ++c) {
+      const double e = emission_E[c] + source_E[c] + (census_E ? census_E[c] : 0.0);
+      s1 += e;
+      s2 += e * e;
+    }
+    h->src_neff_frac = s2 > 0.0 ? (s1 * s1 / s2) / (double)n : 1.0;
+  }
   h->has_energy = true;
   h->has_census_E = census_E != nullptr;
   return BH_OK;
@@ -913,6 +942,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
     stats->lattice = h->lattice ? 1 : 0;
+    stats->tally = h->last_tally;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     stats->census_ms = elapsed_ms(h->ev[1], h->ev[2]);
     stats->source_ms = h->last_source_ms;
@@ -948,7 +978,7 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
           u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
           u(h->h_tally),     (uintptr_t)h->n_cells, (uintptr_t)h->G,
           u(h->rec.p),       u(h->axes.p),    (uintptr_t)h->lattice,
-          (uintptr_t)h->gray_sigma_a, (uintptr_t)h->ctr_hi};
+          (uintptr_t)h->gray_sigma_a, (uintptr_t)use_hash_tally(h), (uintptr_t)h->ctr_hi};
 }
 
 // Records source + transport + census + readback on h->stream (being
@@ -1105,6 +1135,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
     stats->lattice = h->lattice ? 1 : 0;
+    stats->tally = h->last_tally;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph
   }
   if (ctrs.stuck)
@@ -1231,8 +1262,25 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
   const uint64_t saved_hi = h->ctr_hi;
   TRY(upload_aos(h, photons, n, h->C, 0));
   TRY(zero_tallies_and_counters(h));
+  // tally choice from the photons' cells: S evenly spaced samples, effective
+  // number of cells S (S - 1) / sum c (c - 1) (the collision estimator: N for
+  // photons spread evenly over N cells)
+  const double saved_neff = h->src_neff_frac;
+  {
+    const uint64_t S = std::min<uint64_t>(n, 65536);
+    std::vector<uint32_t> cnt(h->n_cells, 0u);
+    for (uint64_t k = 0; k < S; ++k) {
+      const uint32_t c = photons[k * n / S].cell;
+      if (c < h->n_cells) ++cnt[c];
+    }
+    double coll = 0.0;
+    for (const uint32_t v : cnt) coll += (double)v * (double)(v > 0 ? v - 1 : 0);
+    h->src_neff_frac = coll > 0.0 ? (double)S * (double)(S - 1) / coll / (double)h->n_cells : 1.0;
+  }
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-  TRY(launch_transport(h, h->C.soa, n));
+  const int rc_t = launch_transport(h, h->C.soa, n);
+  h->src_neff_frac = saved_neff;
+  TRY(rc_t);
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
   k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(h->C.soa, 0, n, h->ctr_hi, dev_cells(h),
                                                         h->aos.p);
@@ -1261,6 +1309,7 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
     stats->lattice = h->lattice ? 1 : 0;
+    stats->tally = h->last_tally;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     for (uint64_t i = 0; i < n; ++i) {
       const uint8_t d = photons[i].descriptor[0];

@@ -460,22 +460,22 @@ struct GlobalTally {
 // HashTally: the sink of GlobalTally<false> (cont per flush into abs_E, kill
 // weights into track_E, then k_tally_finalize) with the deposits of many
 // flushes combined before they reach HBM.  Scattered FP64 atomics run at
-// ~23e9/s on this device, a tenth of plain stores, and the tally issues one
-// per flush; but the photons of one wave come from consecutive source
-// indices (born in the same few cells) and their flushes keep hitting a
-// small neighbourhood.  Each wave owns an open-addressing table of kSlots
-// (key, value) pairs in LDS: a flush adds into its cell's slot (ds_cmpst to
-// claim, ds_add_f64 to add), and the wave writes every occupied slot out
-// with one global atomic when the table could pass half full, and at exit.
-// Key 0 is empty; cell c's deposit has key c + 1, its kill weights
-// (c + 1) | kKill.  At most kSlots / 2 + 2 * 64 slots are occupied during an
+// ~23e9/s on this device, a tenth of plain stores, and far slower when many
+// of them hit the same few cache lines -- a source face in a thick wall
+// puts nearly every flush of a step into one layer of cells.  The photons of
+// one wave come from consecutive source indices (born in the same few
+// cells), so their flushes keep hitting a small neighbourhood.  Each wave
+// owns an open-addressing table of kSlots cells in LDS: a flush claims its
+// cell's slot (ds_cmpst) and adds its deposit and kill weight
+// (ds_add_f64); the wave writes every occupied slot out with global atomics
+// when the table could pass half full, and at exit.  Key 0 is empty; cell c
+// has key c + 1.  At most kSlots / 2 + 64 slots are occupied during an
 // insertion, so a probe always finds its key or an empty slot.
 struct HashTally {
   static constexpr bool kLds = false;
   static constexpr bool kTrack = false;
   static constexpr uint32_t kSlots = 256;  // per wave, power of two
   static constexpr uint32_t kLimit = kSlots / 2;
-  static constexpr uint32_t kKill = 0x80000000u;
   double* abs_E;
   double* track_E;
   double* s_abs;  // unused
@@ -484,7 +484,8 @@ struct HashTally {
 
   struct Table {
     uint32_t key[kSlots];
-    double val[kSlots];
+    double cont[kSlots];
+    double kill[kSlots];
   };
   __device__ __forceinline__ static Table* table() {
     __shared__ Table t[256 / kWave];
@@ -495,17 +496,17 @@ struct HashTally {
   }
   static_assert(kSlots == 256, "slot_of yields 8 bits");
 
-  __device__ __forceinline__ void clear() {
+  __device__ __forceinline__ void init(uint32_t) {
     Table* t = table();
     const uint32_t lane = threadIdx.x & (kWave - 1);
     for (uint32_t s = lane; s < kSlots; s += kWave) {
       t->key[s] = 0u;
-      t->val[s] = 0.0;
+      t->cont[s] = 0.0;
+      t->kill[s] = 0.0;
     }
     used = 0;
     wave_lds_sync();
   }
-  __device__ __forceinline__ void init(uint32_t) { clear(); }
 
   __device__ __forceinline__ void evict() {
     wave_lds_sync();
@@ -514,42 +515,41 @@ struct HashTally {
     for (uint32_t s = lane; s < kSlots; s += kWave) {
       const uint32_t k = t->key[s];
       if (k != 0u) {
-        double* dst = (k & kKill) ? track_E : abs_E;
-        unsafeAtomicAdd(&dst[(k & ~kKill) - 1u], t->val[s]);
+        unsafeAtomicAdd(&abs_E[k - 1u], t->cont[s]);
+        const double kl = t->kill[s];
+        if (kl != 0.0) unsafeAtomicAdd(&track_E[k - 1u], kl);
         t->key[s] = 0u;
-        t->val[s] = 0.0;
+        t->cont[s] = 0.0;
+        t->kill[s] = 0.0;
       }
     }
     used = 0;
     wave_lds_sync();
   }
 
-  // Every lane at the same program point; lanes with has = false only
-  // take part in the wave-uniform bookkeeping.
-  __device__ __forceinline__ void insert(bool has, uint32_t key, double v) {
-    const uint64_t m = __ballot(has);
+  // Every lane at the same program point; lanes with has = false only take
+  // part in the wave-uniform bookkeeping.
+  __device__ __forceinline__ void push(const Flush& f) {
+    const uint64_t m = __ballot(f.has);
     if (m == 0) return;
     if (used + (uint32_t)__popcll(m) > kLimit) evict();
     Table* t = table();
     bool fresh = false;
-    if (has) {
+    if (f.has) {
+      const uint32_t key = f.c + 1u;
       uint32_t s = slot_of(key);
       for (;;) {
         const uint32_t old = atomicCAS(&t->key[s], 0u, key);
         if (old == 0u || old == key) {
           fresh = old == 0u;
-          unsafeAtomicAdd(&t->val[s], v);
+          unsafeAtomicAdd(&t->cont[s], f.cont);
+          if (f.kill != 0.0) unsafeAtomicAdd(&t->kill[s], f.kill);
           break;
         }
         s = (s + 1u) & (kSlots - 1u);
       }
     }
     used += (uint32_t)__popcll(__ballot(fresh));
-  }
-
-  __device__ __forceinline__ void push(const Flush& f) {
-    insert(f.has, f.c + 1u, f.cont);
-    insert(f.has && f.kill != 0.0, (f.c + 1u) | kKill, f.kill);
   }
   __device__ __forceinline__ void finish(uint32_t) { evict(); }
 };

@@ -557,6 +557,7 @@ int bh_driver_finish(bh_driver* d, bh_driver_report* rep) {
   R.n_flushes = d->stats.n_flushes;
   R.n_replayed = d->stats.n_replayed;
   R.lattice = d->stats.lattice;
+  R.tally = d->stats.tally;
   R.n_exit = d->stats.n_exit;
   R.n_killed = d->stats.n_killed;
   R.emission_E = s.emission_E;

@@ -63,7 +63,11 @@ typedef struct bh_options {
   int32_t block_size;    /* tracking work-items per block: 64, 128 or 256;
                             0 = default (256)                              */
   int32_t blocks_per_cu; /* persistent kernel occupancy, 0 = default        */
-  int32_t tally_mode;    /* 0 auto, 1 global atomics, 2 block-private LDS,
+  int32_t tally_mode;    /* 0 auto (block-private LDS where the mesh fits;
+                            else, for group-independent opacities, mode 4
+                            when the source energy sits in under a quarter
+                            of the cells by (sum e)^2 / sum e^2, mode 1
+                            otherwise), 1 global atomics, 2 block-private LDS,
                             3 global atomics with the track tally summed per
                             segment even for group-independent opacities,
                             4 global atomics of deposits combined per wave
@@ -120,7 +124,9 @@ typedef struct bh_step_stats {
                              finished in FP64 from that segment on        */
   int32_t lattice;        /* 1: tracked on the lattice records (see
                              bh_options.general_mesh), 0: general records */
-  int32_t pad;
+  int32_t tally;          /* tally form of the last transport: 1 one global
+                             atomic per flush, 2 block-private LDS, 3 two
+                             global atomics per flush, 4 wave hash table   */
 } bh_step_stats;
 
 /* A photon in the reference's 120-byte AOS layout (src/photon.h:172-181). */

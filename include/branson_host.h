@@ -47,7 +47,7 @@ typedef struct bh_driver_report {
   double prepare_ms, device_phase_ms, finish_ms; /* host wall clock per phase */
   uint64_t n_replayed;   /* FP32 tracking: histories finished in FP64       */
   int32_t lattice;       /* 1: tracked on the lattice records               */
-  int32_t pad;
+  int32_t tally;         /* tally form (bh_step_stats.tally)                */
 } bh_driver_report;
 
 typedef struct bh_driver bh_driver;
