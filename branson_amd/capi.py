@@ -35,7 +35,8 @@ assert PHOTON_DTYPE.itemsize == 120
 EXPORTS = [
     "bh_abi_version", "bh_create", "bh_destroy", "bh_last_error", "bh_sync", "bh_set_mesh",
     "bh_set_source_energy", "bh_source", "bh_source_from_energy", "bh_set_instances",
-    "bh_source_instances", "bh_transport",
+    "bh_source_instances", "bh_transport", "bh_set_material", "bh_step_resident",
+    "bh_get_temperature", "bh_get_physics",
     "bh_step_graph", "bh_get_tallies", "bh_census_size", "bh_census_export",
     "bh_photons_export",
     "bh_census_import",
@@ -76,6 +77,23 @@ class StepStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Material(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("opA", "opB", "opC", "opS", "cv", "rho", "T_e", "T_r",
+                                          "T_s", "source_face")]
+
+
+class ResidentReport(C.Structure):
+    _fields_ = [("total_E", C.c_double), ("emission_E", C.c_double), ("source_E", C.c_double),
+                ("pre_census_E", C.c_double), ("pre_mat_E", C.c_double),
+                ("post_mat_E", C.c_double), ("absorbed_E", C.c_double), ("stats", StepStats),
+                ("n_new", C.c_uint64), ("etot_serial", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stats"}
+        d["stats"] = self.stats.as_dict()
+        return d
+
+
 _lib = None
 
 
@@ -113,6 +131,10 @@ def lib() -> C.CDLL:
     L.bh_rng_draws.argtypes = [P, U32, U64, U64, U32, P]
     L.bh_threefry.argtypes = [P, P, U64, P]
     L.bh_copy_probe.argtypes = [P, U64, I, P]
+    L.bh_set_material.argtypes = [P, P, D, U32, U32]
+    L.bh_step_resident.argtypes = [P, U64, U32, U32, U32, D, D, P]
+    L.bh_get_temperature.argtypes = [P, P, P]
+    L.bh_get_physics.argtypes = [P, P, P, P, P, P]
     for name in EXPORTS:
         if name not in ("bh_destroy", "bh_last_error", "bh_abi_version"):
             getattr(L, name).restype = I
@@ -296,6 +318,34 @@ class Engine:
                                              _p(df), C.byref(st)))
         return da, df, st.as_dict()
 
+    # -- device-resident step physics -----------------------------------------
+    def set_material(self, opA, opB, opC, opS, cv, rho, T_e, T_r, T_s, source_face,
+                     replicated_factor=1.0, n_ranks=1, rank=0):
+        """bh_set_material: per-cell opacity law, heat capacity, density,
+        initial temperatures and SOURCE face, for bh_step_resident."""
+        n = self.n_cells
+        cols = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,)))
+                for a in (opA, opB, opC, opS, cv, rho, T_e, T_r, T_s)]
+        face = np.ascontiguousarray(np.broadcast_to(np.asarray(source_face, dtype=np.int8),
+                                                    (n,)))
+        m = Material(*[_p(c) for c in cols], _p(face))
+        self._check(lib().bh_set_material(self._h, C.byref(m), replicated_factor, n_ranks,
+                                          rank))
+
+    def step_resident(self, n_user, seed, step, rank, dt, next_dt) -> dict:
+        """bh_step_resident: one whole time step, physics included, on the
+        device.  Returns the report (energies, counts, stats)."""
+        rep = ResidentReport()
+        self._check(lib().bh_step_resident(self._h, n_user, seed, step, rank, dt, next_dt,
+                                           C.byref(rep)))
+        self._n_new = 0
+        return rep.as_dict()
+
+    def temperature(self):
+        T_e, T_r = np.zeros(self.n_cells), np.zeros(self.n_cells)
+        self._check(lib().bh_get_temperature(self._h, _p(T_e), _p(T_r)))
+        return T_e, T_r
+
     # -- RNG probes ----------------------------------------------------------
     def rng_draws(self, seed, stream, n, ctr0=0):
         out = np.zeros(n)
@@ -330,7 +380,7 @@ HOST_EXPORTS = [
 class DriverConfig(C.Structure):
     _fields_ = [("engine", Options), ("device", C.c_int32), ("n_groups", C.c_uint32),
                 ("rank", C.c_uint32), ("n_ranks", C.c_uint32), ("n_photons", C.c_uint64),
-                ("t_stop", C.c_double)]
+                ("t_stop", C.c_double), ("device_physics", C.c_int32), ("reserved", C.c_int32)]
 
 
 class DriverReport(C.Structure):
@@ -396,13 +446,13 @@ class Driver:
                  tally_mode: int = 0, blocks_per_cu: int = 0, refill_threshold: int = 0,
                  reserve_photons: int = 0, fp32_tol: float = 0.0, block_size: int = 0,
                  batch_photons: int = 0, step_graph: bool = False,
-                 general_mesh: bool = False):
+                 general_mesh: bool = False, device_physics: bool = False):
         L = _host_lib()
         self._d = C.c_void_p()
         cfg = DriverConfig(Options(precision, algorithm, block_size, blocks_per_cu, tally_mode,
                                    refill_threshold, reserve_photons, fp32_tol, batch_photons,
                                    int(step_graph), int(general_mesh), 0), device, n_groups,
-                           rank, n_ranks, n_photons, t_stop)
+                           rank, n_ranks, n_photons, t_stop, int(device_physics), 0)
         rc = L.bh_driver_create(os.fsencode(xml_path), C.byref(cfg), C.byref(self._d))
         if rc != BH_OK:
             msg = L.bh_driver_last_error(self._d).decode()

@@ -13,6 +13,7 @@
 
 #include "../../include/branson_hip.h"
 #include "bh_kernels.h"
+#include "bh_physics.h"
 #include "bh_tracking.h"
 
 using namespace bh;
@@ -94,6 +95,18 @@ struct bh_handle {
   bool lattice = false;          // mesh is a lattice: CellRec + axis arrays
   double src_neff_frac = 1.0;    // effective source cells / n_cells (tally choice)
   int last_tally = 0;            // tally form of the last launch (bh_step_stats.tally)
+
+  // device-resident step physics (bh_set_material / bh_step_resident)
+  bool has_material = false;
+  DArr<double> mat;      // [9 n]: opA opB opC opS cv rho T_e T_r T_s
+  DArr<double> mat_e;    // [n] T cV V rho of the cell
+  DArr<double> tr_diag;  // [n] diagnostic radiation temperature
+  double mat_rf = 1.0;
+  uint32_t mat_ranks = 1, mat_rank = 0;
+  PhysTotals* d_phys = nullptr;
+  SourceInst* d_inst2 = nullptr;  // [census, new] source records, E_tot set on the device
+  DArr<double> sum_part;          // block partials of the deterministic sums
+  uint64_t serial_etot_steps = 0; // steps whose E_tot needed the serial sum
   uint32_t dims[3] = {0, 0, 0};
   DArr<CellRec> rec;
   DArr<double> axes;             // x nodes, then y, then z
@@ -602,7 +615,12 @@ void bh_destroy(bh_handle* h) {
   release(h->rep_idx);
   release(h->rep_first);
   release(h->rep_E);
-  void* ptrs[] = {h->W.slab, h->C.slab, h->d_total, h->d_ctot, h->d_ctrs, h->d_queue};
+  release(h->mat);
+  release(h->mat_e);
+  release(h->tr_diag);
+  release(h->sum_part);
+  void* ptrs[] = {h->W.slab, h->C.slab, h->d_total, h->d_ctot, h->d_ctrs, h->d_queue,
+                  h->d_phys, h->d_inst2};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& g : h->graphs) {
@@ -746,19 +764,28 @@ int bh_set_source_energy(bh_handle* h, const double* emission_E, const double* s
 // The step's source for one problem (inst_src null: `census` / `fresh`
 // describe it) or for every instance (inst_src[0, n_inst) census,
 // [n_inst, 2 n_inst) new photons, on the device).
+// icen / inew: source parameter records on the device (per instance, or one
+// record whose E_tot the device computed), else `census` / `fresh` are used.
+// census_part: step 1 of a resident step, block partials of the initial
+// census energy go there (census_nb blocks).
 static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& fresh,
-                      const SourceInst* inst_src, double dt, uint64_t* n_new_out) {
+                      const SourceInst* icen, const SourceInst* inew, double dt,
+                      uint64_t* n_new_out, double* census_part = nullptr,
+                      uint32_t census_nb = 0) {
   const uint32_t n = h->n_cells;
   const DevCells cells = dev_cells(h);
   TRY(ensure(h, h->counts, n));
   TRY(ensure(h, h->off, n));
   const uint32_t* cinst = h->n_inst ? h->cell_inst.p : nullptr;
-  const SourceInst* icen = inst_src;
-  const SourceInst* inew = inst_src ? inst_src + h->n_inst : nullptr;
   HIPCHK(h, hipEventRecord(h->ev[4], h->stream));
   if (h->has_census_E) {  // step 1: initial census replaces the resident one
     k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->cen.p, nullptr, n, census, icen,
                                                               cinst, 1, h->counts.p);
+    if (census_part) {  // initial census energy, block partials
+      k_census_partials<<<census_nb, kSumNT, 0, h->stream>>>(h->cen.p, h->counts.p, n,
+                                                             census_part);
+      HIPCHK(h, hipGetLastError());
+    }
     TRY(scan_u64(h, h->counts.p, n, h->off.p, 0));
     uint64_t n_c = 0;
     TRY(read_total(h, 0, &n_c));
@@ -808,7 +835,7 @@ int bh_source(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_t se
   const SourceInst census{n_user_photons, total_E, rank_off, ctr_hi, 0, 0};
   const SourceInst fresh{n_user_photons, total_E, 10000000000000ull * (uint64_t)cycle + rank_off,
                          ctr_hi, 0, 0};
-  return run_source(h, census, fresh, nullptr, dt, n_new_out);
+  return run_source(h, census, fresh, nullptr, nullptr, dt, n_new_out);
 }
 
 int bh_set_instances(bh_handle* h, uint32_t n_instances, const uint32_t* first_cell,
@@ -870,7 +897,7 @@ int bh_source_instances(bh_handle* h, const uint64_t* n_user_photons, const doub
   HIPCHK(h, hipMemcpyAsync(h->inst_src.p, p.data(), p.size() * sizeof(SourceInst),
                            hipMemcpyHostToDevice, h->stream));
   const SourceInst unused{};
-  return run_source(h, unused, unused, h->inst_src.p, dt, n_new_out);
+  return run_source(h, unused, unused, h->inst_src.p, h->inst_src.p + h->n_inst, dt, n_new_out);
 }
 
 int bh_source_from_energy(bh_handle* h, const double* emission_E, const double* source_E,
@@ -1387,6 +1414,218 @@ int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
   if (ctrs.stuck)
     return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
                 ctrs.stuck, kMaxSegments);
+  return BH_OK;
+}
+
+// ---- device-resident step physics -------------------------------------------
+static MatDev mat_dev(bh_handle* h) {
+  const size_t n = h->n_cells;
+  double* b = h->mat.p;
+  return MatDev{b,         b + n,     b + 2 * n, b + 3 * n, b + 4 * n,
+                b + 5 * n, b + 6 * n, b + 7 * n, b + 8 * n, h->face.p};
+}
+
+int bh_set_material(bh_handle* h, const bh_material* m, double replicated_factor,
+                    uint32_t n_ranks, uint32_t rank) {
+  if (!h || !h->stream || !m) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_material before bh_set_mesh");
+  if (h->n_inst) return fail(h, BH_ERR_STATE, "resident step physics runs one problem");
+  if (n_ranks == 0 || rank >= n_ranks || !(replicated_factor > 0.0))
+    return fail(h, BH_ERR_ARG, "need n_ranks >= 1, rank < n_ranks, replicated_factor > 0");
+  const double* cols[9] = {m->opA, m->opB, m->opC, m->opS, m->cv, m->rho, m->T_e, m->T_r, m->T_s};
+  for (const double* c : cols)
+    if (!c) return fail(h, BH_ERR_ARG, "every bh_material array is required");
+  if (!m->source_face) return fail(h, BH_ERR_ARG, "every bh_material array is required");
+  const uint32_t n = h->n_cells;
+  for (uint32_t c = 0; c < n; ++c) {
+    if (m->source_face[c] < -1 || m->source_face[c] > 5)
+      return fail(h, BH_ERR_ARG, "cell %u: source_face must be -1..5", c);
+    if (!(m->cv[c] > 0.0) || !(m->rho[c] > 0.0))
+      return fail(h, BH_ERR_ARG, "cell %u: cv and rho must be > 0", c);
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  TRY(ensure(h, h->mat, (size_t)9 * n));
+  TRY(ensure(h, h->mat_e, n));
+  TRY(ensure(h, h->tr_diag, n));
+  TRY(ensure(h, h->em, n));
+  TRY(ensure(h, h->src, n));
+  TRY(ensure(h, h->cen, n));
+  TRY(ensure(h, h->face, n));
+  if (!h->d_phys) HIPCHK(h, hipMalloc(&h->d_phys, sizeof(PhysTotals)));
+  if (!h->d_inst2) HIPCHK(h, hipMalloc(&h->d_inst2, 2 * sizeof(SourceInst)));
+  for (int k = 0; k < 9; ++k)
+    HIPCHK(h, hipMemcpyAsync(h->mat.p + (size_t)k * n, cols[k], n * sizeof(double),
+                             hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->face.p, m->source_face, n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->tr_diag.p, 0, n * sizeof(double), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->mat_rf = replicated_factor;
+  h->mat_ranks = n_ranks;
+  h->mat_rank = rank;
+  h->has_material = true;
+  return BH_OK;
+}
+
+int bh_step_resident(bh_handle* h, uint64_t n_user_photons, uint32_t seed, uint32_t step,
+                     uint32_t rank, double dt, double next_dt, bh_resident_report* rep) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh || !h->has_material)
+    return fail(h, BH_ERR_STATE, "bh_step_resident needs bh_set_mesh and bh_set_material");
+  if (h->n_inst) return fail(h, BH_ERR_STATE, "resident step physics runs one problem");
+  if (step == 0) return fail(h, BH_ERR_ARG, "step counts from 1");
+  if (n_user_photons == 0 || n_user_photons > 0xFFFFFFFFull)
+    return fail(h, BH_ERR_ARG, "n_user_photons must be in [1, 2^32)");
+  if (!(dt > 0.0)) return fail(h, BH_ERR_ARG, "dt must be > 0");
+  HIPCHK(h, hipSetDevice(h->device));
+  // the step's physics fills every group alike (Cell::set_op_a / set_op_s)
+  h->uniform_sigma_a = true;
+  h->gray_sigma_a = h->opt.tally_mode != 3;
+  const uint64_t ctr_hi = (uint64_t)seed << 32;
+  if (h->n_cen && step > 1 && ctr_hi != h->ctr_hi)
+    return fail(h, BH_ERR_ARG, "seed differs from the resident census photons' seed");
+  h->ctr_hi = ctr_hi;
+  const uint32_t n = h->n_cells;
+  const bool first = step == 1;
+  const MatDev md = mat_dev(h);
+  // Mesh::calculate_photon_energy
+  k_phys_cells<<<grid_for(n, 256), 256, 0, h->stream>>>(
+      h->geo.p, h->sigma.p, h->lattice ? h->rec.p : nullptr, h->G, n, md, dt, h->mat_rf,
+      first ? 1 : 0, h->em.p, h->src.p, h->cen.p, h->mat_e.p);
+  HIPCHK(h, hipGetLastError());
+  const uint64_t rank_off = n_user_photons * (uint64_t)rank;
+  const SourceInst host2[2] = {
+      SourceInst{n_user_photons, 0.0, rank_off, ctr_hi, 0, 0},
+      SourceInst{n_user_photons, 0.0, 10000000000000ull * (uint64_t)step + rank_off, ctr_hi, 0,
+                 0}};
+  char* hp = static_cast<char*>(h->h_pinned);
+  std::memcpy(hp + 1024, host2, sizeof host2);
+  HIPCHK(h, hipMemcpyAsync(h->d_inst2, hp + 1024, sizeof host2, hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_phys, 0, sizeof(PhysTotals), h->stream));
+  const uint32_t nb = std::max<uint32_t>(1u, std::min<uint32_t>(kSumMaxBlocks, div_up(n, 4096)));
+  TRY(ensure(h, h->sum_part, (size_t)8 * nb));
+  const bool serial = h->mat_ranks > 1;
+  if (!serial) {
+    // E_tot in parallel with a bound on the serial loop's value, and a check
+    // that no photon count depends on where in the bound it lies
+    k_phys_partials<<<nb, kSumNT, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
+                                                  h->sum_part.p);
+    k_final_sums<7><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->tot_emission);
+    k_phys_etot<<<1, 1, 0, h->stream>>>(n, h->d_phys, h->d_inst2);
+    k_count_check<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, n,
+                                                           n_user_photons, first ? 1 : 0,
+                                                           h->d_phys);
+    HIPCHK(h, hipGetLastError());
+  }
+  PhysTotals pt;
+  HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::memcpy(&pt, hp + 2048, sizeof pt);
+  const bool used_serial = pt.ambiguous != 0;
+  if (serial || used_serial) {  // the reference's serial cell-order sums
+    k_phys_totals<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
+                                              h->mat_ranks, 0, h->d_phys, h->d_inst2);
+    HIPCHK(h, hipGetLastError());
+    if (h->mat_ranks > 1) {  // with one rank the reassignment leaves every energy as it is
+      k_phys_reassign<<<grid_for(n, 256), 256, 0, h->stream>>>(
+          h->em.p, h->src.p, h->cen.p, n, n_user_photons, h->mat_ranks, h->mat_rank, h->mat_rf,
+          first ? 1 : 0, h->d_phys);
+      k_phys_totals<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
+                                                h->mat_ranks, 1, h->d_phys, h->d_inst2);
+      HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::memcpy(&pt, hp + 2048, sizeof pt);
+    ++h->serial_etot_steps;
+  }
+  if (!(pt.global_E > 0.0)) return fail(h, BH_ERR_STATE, "no photon energy this step");
+  h->src_neff_frac = pt.e_sq > 0.0 ? (pt.e_sum * pt.e_sum / pt.e_sq) / (double)n : 1.0;
+  h->has_energy = true;
+  h->has_census_E = first;
+  // source: initial census at step 1, then this step's photons
+  uint64_t n_new = 0;
+  TRY(run_source(h, host2[0], host2[1], h->d_inst2, h->d_inst2 + 1, dt, &n_new,
+                 first ? h->sum_part.p : nullptr, nb));
+  if (first) {
+    k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->pre_census);
+    HIPCHK(h, hipGetLastError());
+  }
+  const uint64_t n_transported = h->n_cen + n_new;
+  // transport + census; the tallies stay on the device
+  bh_step_stats st;
+  TRY(bh_transport(h, next_dt, nullptr, nullptr, &st));
+  // Mesh::update_temperature
+  k_update_temperature<<<grid_for(n, 256), 256, 0, h->stream>>>(
+      h->geo.p, md, h->abs_E.p, h->track_E.p, h->em.p, dt, n, h->tr_diag.p, h->mat_e.p);
+  HIPCHK(h, hipGetLastError());
+  k_report_partials<<<nb, kSumNT, 0, h->stream>>>(h->abs_E.p, h->mat_e.p, n, h->sum_part.p);
+  k_final_sums<2><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->absorbed);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::memcpy(&pt, hp + 2048, sizeof pt);
+  if (rep) {
+    std::memset(rep, 0, sizeof *rep);
+    rep->total_E = pt.global_E;
+    rep->emission_E = pt.tot_emission;
+    rep->source_E = pt.tot_source;
+    rep->pre_census_E = first ? pt.pre_census : 0.0;
+    rep->pre_mat_E = pt.pre_mat;
+    rep->post_mat_E = pt.post_mat;
+    rep->absorbed_E = pt.absorbed;
+    rep->stats = st;
+    rep->stats.n_transported = n_transported;
+    rep->n_new = n_new;
+    rep->etot_serial = (serial || used_serial) ? 1 : 0;
+  }
+  return BH_OK;
+}
+
+int bh_get_temperature(bh_handle* h, double* T_e, double* T_r) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_material) return fail(h, BH_ERR_STATE, "no material (bh_set_material)");
+  const size_t n = h->n_cells;
+  if (T_e)
+    HIPCHK(h, hipMemcpyAsync(T_e, h->mat.p + 6 * n, n * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+  if (T_r)
+    HIPCHK(h, hipMemcpyAsync(T_r, h->tr_diag.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return BH_OK;
+}
+
+int bh_get_physics(bh_handle* h, double* sigma_a, double* sigma_s, double* fleck,
+                   double* emission_E, double* source_E) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "no mesh");
+  const size_t n = h->n_cells, G = h->G;
+  std::vector<double2> sig;
+  std::vector<CellGeo> geo;
+  if (sigma_a || sigma_s) {
+    sig.resize(n * G);
+    HIPCHK(h, hipMemcpyAsync(sig.data(), h->sigma.p, n * G * sizeof(double2),
+                             hipMemcpyDeviceToHost, h->stream));
+  }
+  if (fleck) {
+    geo.resize(n);
+    HIPCHK(h, hipMemcpyAsync(geo.data(), h->geo.p, n * sizeof(CellGeo), hipMemcpyDeviceToHost,
+                             h->stream));
+  }
+  if (emission_E && h->em.p)
+    HIPCHK(h, hipMemcpyAsync(emission_E, h->em.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+  if (source_E && h->src.p)
+    HIPCHK(h, hipMemcpyAsync(source_E, h->src.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t k = 0; k < sig.size(); ++k) {
+    if (sigma_a) sigma_a[k] = sig[k].x;
+    if (sigma_s) sigma_s[k] = sig[k].y;
+  }
+  for (size_t c = 0; c < geo.size(); ++c) fleck[c] = geo[c].fleck;
   return BH_OK;
 }
 

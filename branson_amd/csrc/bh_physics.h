@@ -1,0 +1,313 @@
+// bh_physics.h -- a time step's physics on the device, so that temperatures,
+// opacities, source energies and tallies never leave HBM between steps:
+//   Mesh::calculate_photon_energy (src/mesh.h:237-323) -> k_phys_cells +
+//     k_phys_partials + k_phys_etot + k_count_check (k_phys_totals serially),
+//   Mesh::update_temperature (src/mesh.h:327-423) -> k_update_temperature +
+//     k_report_partials.
+// Every formula keeps the reference's FP64 operation order (the build uses
+// -ffp-contract=off).  Sums run in the reference's cell order: the photon
+// counts int(n_user * E_c / E_tot) depend on the last bit of E_tot, so E_tot
+// must give the counts of the reference's serial sum: it is summed in
+// parallel with an error bound, and serially (exactly as the reference's
+// loop) only when a count could depend on the difference.
+#pragma once
+
+#include "bh_kernels.h"
+
+namespace bh {
+
+// Per-cell material (Region, src/region.h:27-76) and state, device arrays.
+struct MatDev {
+  const double *opA, *opB, *opC, *opS, *cv, *rho;
+  double* T_e;        // electron temperature, updated every step
+  const double* T_r;  // radiation temperature of the initial census
+  const double* T_s;  // temperature of the cell's SOURCE face
+  const int8_t* face;
+};
+
+// Step totals the host reads back (one per engine, in device memory).  The
+// first seven are the sums of calculate_photon_energy's loop, in the order
+// k_phys_partials produces them.
+struct PhysTotals {
+  double tot_emission, tot_census, tot_source;  // after the small-energy pass
+  double total_photon_E;                        // this rank's photon energy
+  double pre_mat;                               // sum T cV V rho before the step
+  double e_sum, e_sq;                           // sum e, sum e^2 of e = em + src + cen
+  double absorbed, post_mat;                    // after update_temperature
+  double pre_census;                            // step 1: initial census energy
+  double global_E;                              // E_tot of the photon counts
+  double E_lo, E_hi;                            // bounds on the serial-order E_tot
+  uint32_t ambiguous;                           // a count could differ within them
+  uint32_t pad;
+};
+
+// ---- deterministic parallel sums ------------------------------------------
+// Block b sums a fixed range of cells, each work-item a fixed stride of it,
+// then a fixed butterfly within the wave and the four waves in order; the
+// block partials are summed the same way by one block.  The result does not
+// depend on scheduling, so a run is reproducible, but it is not the
+// reference's serial rounding (see k_count_check).
+constexpr int kSumNT = 256;
+constexpr int kSumMaxBlocks = 1024;
+
+template <int K>
+__device__ __forceinline__ void block_sums(double (&v)[K]) {
+  __shared__ double lds[K][kSumNT / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v[k] += __shfl_xor(v[k], d, kWave);
+    if (lane == 0) lds[k][wave] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = ((lds[k][0] + lds[k][1]) + lds[k][2]) + lds[k][3];
+}
+
+__device__ __forceinline__ void sum_range(uint32_t n, uint32_t& lo, uint32_t& hi) {
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  lo = blockIdx.x * per;
+  hi = lo + per < n ? lo + per : n;
+}
+
+// The seven sums of calculate_photon_energy's loop, one partial per block.
+__global__ __launch_bounds__(kSumNT) void k_phys_partials(const double* __restrict__ em,
+                                                          const double* __restrict__ src,
+                                                          const double* __restrict__ cen,
+                                                          const double* __restrict__ mat,
+                                                          uint32_t n, double* __restrict__ part) {
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t lo, hi;
+  sum_range(n, lo, hi);
+  for (uint32_t c = lo + threadIdx.x; c < hi; c += kSumNT) {
+    const double e = em[c], s = src[c], ce = cen[c];
+    v[0] += e;
+    v[1] += ce;
+    v[2] += s;
+    v[3] += s + ce + e;
+    v[4] += mat[c];
+    const double all = e + s + ce;
+    v[5] += all;
+    v[6] += all * all;
+  }
+  block_sums<7>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 7; ++k) part[blockIdx.x * 8 + k] = v[k];
+}
+
+// absorbed energy and post-step material energy, one partial per block
+__global__ __launch_bounds__(kSumNT) void k_report_partials(const double* __restrict__ abs_E,
+                                                            const double* __restrict__ mat,
+                                                            uint32_t n, double* __restrict__ part) {
+  double v[2] = {0, 0};
+  uint32_t lo, hi;
+  sum_range(n, lo, hi);
+  for (uint32_t c = lo + threadIdx.x; c < hi; c += kSumNT) {
+    v[0] += abs_E[c];
+    v[1] += mat[c];
+  }
+  block_sums<2>(v);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 8] = v[0];
+    part[blockIdx.x * 8 + 1] = v[1];
+  }
+}
+
+// Step 1: energy of the initial census photons, n_c copies of E_c / n_c per
+// cell (each cell's copies as one product), one partial per block.
+__global__ __launch_bounds__(kSumNT) void k_census_partials(const double* __restrict__ cen,
+                                                            const uint64_t* __restrict__ counts,
+                                                            uint32_t n, double* __restrict__ part) {
+  double v[1] = {0};
+  uint32_t lo, hi;
+  sum_range(n, lo, hi);
+  for (uint32_t c = lo + threadIdx.x; c < hi; c += kSumNT) {
+    const uint64_t k = counts[c];
+    if (k > 0) v[0] += (double)k * (cen[c] / (double)k);
+  }
+  block_sums<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x * 8] = v[0];
+}
+
+// One block: dst[k] = sum over the nb block partials of quantity k.
+template <int K>
+__global__ __launch_bounds__(kSumNT) void k_final_sums(const double* __restrict__ part,
+                                                       uint32_t nb, double* __restrict__ dst) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (uint32_t b = threadIdx.x; b < nb; b += kSumNT)
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += part[b * 8 + k];
+  block_sums<K>(v);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) dst[k] = v[k];
+}
+
+// One rank: E_tot is this rank's total, and the serial loop's value lies
+// within E_par +- delta, delta = 2 (n + 64) u E_par (u = 2^-53; the summands
+// are the same non-negative doubles on both sides: the serial sum is within
+// (n - 1) u sum x of the exact sum, the parallel one within (log2 n + 4) u).
+__global__ void k_phys_etot(uint32_t n, PhysTotals* __restrict__ out,
+                            SourceInst* __restrict__ inst2) {
+  const double E = out->total_photon_E;
+  const double delta = 2.0 * ((double)n + 64.0) * 0x1p-53 * E;
+  out->global_E = E;
+  out->E_lo = E - delta;
+  out->E_hi = E + delta;
+  out->ambiguous = 0u;
+  inst2[0].total_E = E;
+  inst2[1].total_E = E;
+}
+
+// Per cell: does int(n_user * E_c / E_tot) (the photon count, monotone in
+// E_tot) take the same value at both ends of [E_lo, E_hi]?  If it does for
+// every energy of every cell, the counts are those of the reference's serial
+// E_tot, whatever its last bits; if not, the engine computes E_tot serially.
+__global__ __launch_bounds__(256) void k_count_check(const double* __restrict__ em,
+                                                     const double* __restrict__ src,
+                                                     const double* __restrict__ cen,
+                                                     uint32_t n, uint64_t n_user, int first_step,
+                                                     PhysTotals* __restrict__ out) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double lo = out->E_lo, hi = out->E_hi;
+  const double e3[3] = {em[c], src[c], first_step ? cen[c] : 0.0};
+  bool amb = false;
+  for (int k = 0; k < 3; ++k)
+    if (e3[k] > 0.0) amb |= photons_for(n_user, e3[k], lo) != photons_for(n_user, e3[k], hi);
+  if (amb) atomicOr(&out->ambiguous, 1u);
+}
+
+__device__ __forceinline__ double cell_volume(const CellGeo& g) {
+  return (g.nodes[1] - g.nodes[0]) * (g.nodes[3] - g.nodes[2]) * (g.nodes[5] - g.nodes[4]);
+}
+
+// Cell::get_face_area (src/cell.h:83-100); -1 for "no face".
+__device__ __forceinline__ double face_area(const CellGeo& g, int face) {
+  if (face == 0 || face == 1) return (g.nodes[3] - g.nodes[2]) * (g.nodes[5] - g.nodes[4]);
+  if (face == 2 || face == 3) return (g.nodes[1] - g.nodes[0]) * (g.nodes[5] - g.nodes[4]);
+  if (face == 4 || face == 5) return (g.nodes[1] - g.nodes[0]) * (g.nodes[3] - g.nodes[2]);
+  return -1.0;
+}
+
+// Per cell: opacities and Fleck factor into the tracking records (every
+// group, Cell::set_op_a / set_op_s, src/cell.h:260-275), and the emission,
+// initial-census and face-source energies (src/mesh.h:257-281).
+__global__ __launch_bounds__(256) void k_phys_cells(CellGeo* __restrict__ geo,
+                                                    double2* __restrict__ sigma,
+                                                    CellRec* __restrict__ rec, uint32_t G,
+                                                    uint32_t n, MatDev m, double dt, double rf,
+                                                    int first_step, double* __restrict__ em,
+                                                    double* __restrict__ src,
+                                                    double* __restrict__ cen,
+                                                    double* __restrict__ mat) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CellGeo g = geo[i];
+  const double vol = cell_volume(g);
+  const double cV = m.cv[i], T = m.T_e[i], Tr = m.T_r[i], Ts = m.T_s[i], rho = m.rho[i];
+  const double op_a = m.opA[i] + m.opB[i] * pow(T, m.opC[i]);  // src/region.h:44-46
+  const double op_s = m.opS[i];
+  const double f = 1.0 / (1.0 + dt * op_a * kC * (4.0 * kA * pow(T, 3.0) / (cV * rho)));
+  for (uint32_t k = 0; k < G; ++k) sigma[(size_t)i * G + k] = make_double2(op_a, op_s);
+  geo[i].fleck = f;
+  if (rec) {
+    rec[i].sigma_a = op_a;
+    rec[i].sigma_s = op_s;
+    rec[i].fleck = f;
+  }
+  em[i] = rf * dt * vol * f * op_a * kA * kC * pow(T, 4.0);
+  cen[i] = first_step ? rf * vol * kA * pow(Tr, 4.0) : 0.0;
+  src[i] = rf * 0.25 * kA * kC * face_area(g, m.face[i]) * pow(Ts, 4.0) * dt;
+  mat[i] = T * cV * vol * rho;
+}
+
+// Serial cell-order sums, the reference loop's exact rounding (the fallback
+// when k_count_check finds a count that E_tot's last bits could change, and
+// the path for more than one rank).  Every lane of one wave walks the cells
+// in order with the same (uniform) index: scalar loads, one dependent chain
+// of FP64 adds per sum.
+//
+// Pass 1 (pass = 0): the sums of calculate_photon_energy's loop, E_tot for the
+// photon counts -- this rank's total with one rank; with several, n_ranks
+// times the local sum, the small-energy reassignment (k_phys_reassign) and
+// pass 2 (pass = 1), which recomputes the step totals only -- and the source
+// parameter records' E_tot.
+__global__ __launch_bounds__(kWave) void k_phys_totals(const double* __restrict__ em,
+                                                       const double* __restrict__ src,
+                                                       const double* __restrict__ cen,
+                                                       const double* __restrict__ mat,
+                                                       uint32_t n, uint32_t n_ranks, int pass,
+                                                       PhysTotals* __restrict__ out,
+                                                       SourceInst* __restrict__ inst2) {
+  double t_em = 0.0, t_cen = 0.0, t_src = 0.0, total = 0.0, pre_mat = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll 8
+  for (uint32_t c = 0; c < n; ++c) {
+    const double e = em[c], s = src[c], ce = cen[c];
+    t_em += e;
+    t_cen += ce;
+    t_src += s;
+    total += s + ce + e;
+    if (pass == 0) {
+      pre_mat += mat[c];
+      const double all = e + s + ce;
+      e1 += all;
+      e2 += all * all;
+    }
+  }
+  if (threadIdx.x != 0) return;
+  out->tot_emission = t_em;
+  out->tot_census = t_cen;
+  out->tot_source = t_src;
+  out->total_photon_E = total;
+  if (pass == 0) {
+    out->global_E = (n_ranks == 1) ? total : (double)n_ranks * (t_em + t_cen + t_src);
+    out->pre_mat = pre_mat;
+    out->e_sum = e1;
+    out->e_sq = e2;
+    inst2[0].total_E = out->global_E;
+    inst2[1].total_E = out->global_E;
+  }
+}
+
+// Replicated small-energy reassignment (src/mesh.h:291-315), more than one
+// rank: a cell whose energy would get no photon of its own from the global
+// n_user (int(n_user * E / global_E) == 0) is given whole to rank
+// cell % n_ranks.
+__global__ __launch_bounds__(256) void k_phys_reassign(double* __restrict__ em,
+                                                       double* __restrict__ src,
+                                                       double* __restrict__ cen, uint32_t n,
+                                                       uint64_t n_user, uint32_t n_ranks,
+                                                       uint32_t rank, double rf, int first_step,
+                                                       const PhysTotals* __restrict__ tot) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double global_E = tot->global_E;
+  const double nu = (double)(uint32_t)n_user;
+  const bool mine = (c % n_ranks) == rank;
+  const double ce = cen[c], e = em[c], s = src[c];
+  if (first_step && ce > 0.0 && (int)(nu * (ce / global_E)) == 0) cen[c] = mine ? ce / rf : 0.0;
+  if (e > 0.0 && (int)(nu * (e / global_E)) == 0) em[c] = mine ? e / rf : 0.0;
+  if (s > 0.0 && (int)(nu * (s / global_E)) == 0) src[c] = mine ? s / rf : 0.0;
+}
+
+// Mesh::update_temperature, one rank (src/mesh.h:327-423): new electron
+// temperature from the step's absorbed and emitted energy, the diagnostic
+// radiation temperature from the track tally, and T cV V rho per cell.
+__global__ __launch_bounds__(256) void k_update_temperature(
+    const CellGeo* __restrict__ geo, MatDev m, const double* __restrict__ abs_E,
+    const double* __restrict__ track_E, const double* __restrict__ em, double dt, uint32_t n,
+    double* __restrict__ T_r_out, double* __restrict__ mat) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double cV = m.cv[i], rho = m.rho[i], vol = cell_volume(geo[i]), T = m.T_e[i];
+  const double T_new = T + (abs_E[i] - em[i]) / (cV * vol * rho);
+  T_r_out[i] = pow(track_E[i] / (vol * dt * kA * kC), 0.25);
+  m.T_e[i] = T_new;
+  mat[i] = T_new * cV * vol * rho;
+}
+
+}  // namespace bh

@@ -316,6 +316,8 @@ struct bh_driver {
   uint64_t census_size = 0;
   enum Phase { kIdle, kPrepared, kTransported } phase = kIdle;
   bh_step_stats stats{};
+  bh_resident_report rr{};      // device_physics: the last resident step
+  bool resident_ready = false;  // device_physics: geometry and material uploaded
   uint64_t transported = 0;
   double prepare_ms = 0, device_ms = 0;
 };
@@ -483,6 +485,36 @@ int bh_driver_prepare(bh_driver* d) {
   const auto t0 = std::chrono::steady_clock::now();
   int rce = ensure_engine(d);
   if (rce != BH_OK) return rce;
+  if (d->cfg.device_physics) {  // geometry and material once; physics on the device
+    if (!d->resident_ready) {
+      calculate_photon_energy(d);  // first opacities: the lattice test needs group-uniform ones
+      Mesh& m = d->mesh;
+      bh_cells c{m.n, m.G, m.nodes.data(), m.bc.data(), m.next.data(),
+                 m.sigma_a.data(), m.sigma_s.data(), m.fleck.data()};
+      int rc = bh_set_mesh(d->eng, &c);
+      if (rc != BH_OK) return engine_fail(d, rc);
+      std::vector<double> col[6];
+      for (auto& v : col) v.resize(m.n);
+      for (uint32_t i = 0; i < m.n; ++i) {
+        const Region& r = d->deck.regions[m.region[i]];
+        col[0][i] = r.opA;
+        col[1][i] = r.opB;
+        col[2][i] = r.opC;
+        col[3][i] = r.opS;
+        col[4][i] = m.cv[i];
+        col[5][i] = m.rho[i];
+      }
+      bh_material mat{col[0].data(), col[1].data(), col[2].data(), col[3].data(),
+                      col[4].data(), col[5].data(), m.T_e.data(),  m.T_r_cell.data(),
+                      m.T_s.data(),  m.source_face.data()};
+      rc = bh_set_material(d->eng, &mat, m.replicated_factor, d->cfg.n_ranks, d->cfg.rank);
+      if (rc != BH_OK) return engine_fail(d, rc);
+      d->resident_ready = true;
+    }
+    d->prepare_ms = ms_since(t0);
+    d->phase = bh_driver::kPrepared;
+    return BH_OK;
+  }
   calculate_photon_energy(d);
   Mesh& m = d->mesh;
   bh_cells c{m.n, m.G, m.nodes.data(), m.bc.data(), m.next.data(),
@@ -501,6 +533,21 @@ int bh_driver_transport(bh_driver* d) {
   if (!d || !d->eng) return BH_ERR_ARG;
   if (d->phase != bh_driver::kPrepared) return dfail(d, BH_ERR_STATE, "call bh_driver_prepare first");
   const auto t0 = std::chrono::steady_clock::now();
+  if (d->cfg.device_physics) {  // the whole step, physics included, on the device
+    const int rc = bh_step_resident(d->eng, d->deck.n_photons, d->deck.seed, d->st.step,
+                                    d->cfg.rank, d->st.dt, d->st.next_dt(), &d->rr);
+    if (rc != BH_OK) return engine_fail(d, rc);
+    d->stats = d->rr.stats;
+    d->transported = d->rr.stats.n_transported;
+    d->st.pre_census_E = (d->st.step == 1) ? d->rr.pre_census_E : d->prev_census_E;
+    d->st.pre_mat_E = d->rr.pre_mat_E;
+    d->st.emission_E = d->rr.emission_E;
+    d->st.source_E = d->rr.source_E;
+    d->mesh.global_photon_E = d->rr.total_E;
+    d->device_ms = ms_since(t0);
+    d->phase = bh_driver::kTransported;
+    return BH_OK;
+  }
   const double total_E = d->mesh.global_photon_E;
   if (!(total_E > 0.0)) return dfail(d, BH_ERR_STATE, "no photon energy this step");
   // pre-transport census energy (src/replicated_driver.h:61, 71)
@@ -531,12 +578,16 @@ int bh_driver_finish(bh_driver* d, bh_driver_report* rep) {
   if (!d || !d->eng) return BH_ERR_ARG;
   if (d->phase != bh_driver::kTransported) return dfail(d, BH_ERR_STATE, "call bh_driver_transport first");
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = bh_get_tallies(d->eng, d->abs_E.data(), d->track_E.data());
-  if (rc != BH_OK) return engine_fail(d, rc);
-  // Mesh::update_temperature (src/mesh.h:327-423), one rank
   Mesh& m = d->mesh;
   State& s = d->st;
   double total_abs = 0, post_mat = 0;
+  if (d->cfg.device_physics) {  // updated on the device by bh_step_resident
+    total_abs = d->rr.absorbed_E;
+    post_mat = d->rr.post_mat_E;
+  } else {
+  int rc = bh_get_tallies(d->eng, d->abs_E.data(), d->track_E.data());
+  if (rc != BH_OK) return engine_fail(d, rc);
+  // Mesh::update_temperature (src/mesh.h:327-423), one rank
   for (uint32_t i = 0; i < m.n; ++i) {
     const Region& r = d->deck.regions[m.region[i]];
     const double cV = r.cv, rho = r.rho, vol = m.volume(i), T = m.T_e[i];
@@ -545,6 +596,7 @@ int bh_driver_finish(bh_driver* d, bh_driver_report* rep) {
     m.T_e[i] = T_new;
     total_abs += d->abs_E[i];
     post_mat += T_new * cV * vol * rho;
+  }
   }
   bh_driver_report R{};
   R.step = s.step;
@@ -595,6 +647,7 @@ int bh_driver_step(bh_driver* d, bh_driver_report* rep) {
 
 int bh_driver_get_tallies(const bh_driver* d, double* abs_E, double* track_E) {
   if (!d) return BH_ERR_ARG;
+  if (d->cfg.device_physics && d->eng) return bh_get_tallies(d->eng, abs_E, track_E);
   if (abs_E) std::memcpy(abs_E, d->abs_E.data(), d->mesh.n * sizeof(double));
   if (track_E) std::memcpy(track_E, d->track_E.data(), d->mesh.n * sizeof(double));
   return BH_OK;
@@ -602,6 +655,7 @@ int bh_driver_get_tallies(const bh_driver* d, double* abs_E, double* track_E) {
 
 int bh_driver_get_temperature(const bh_driver* d, double* T_e, double* T_r) {
   if (!d) return BH_ERR_ARG;
+  if (d->cfg.device_physics && d->resident_ready) return bh_get_temperature(d->eng, T_e, T_r);
   if (T_e) std::memcpy(T_e, d->mesh.T_e.data(), d->mesh.n * sizeof(double));
   if (T_r) std::memcpy(T_r, d->mesh.T_r.data(), d->mesh.n * sizeof(double));
   return BH_OK;
@@ -624,6 +678,14 @@ int bh_driver_get_physics(const bh_driver* d, double* sigma_a, double* sigma_s, 
   if (!d) return BH_ERR_ARG;
   const Mesh& m = d->mesh;
   const size_t n = m.n;
+  if (d->cfg.device_physics && d->resident_ready) {
+    const int rc = bh_get_physics(d->eng, sigma_a, sigma_s, fleck, emission_E, source_E);
+    if (rc != BH_OK) return rc;
+    if (census_E) std::memset(census_E, 0, n * sizeof(double));
+    if (source_face) std::memcpy(source_face, m.source_face.data(), n);
+    if (total_E) *total_E = 0.0;
+    return BH_OK;
+  }
   if (sigma_a) std::memcpy(sigma_a, m.sigma_a.data(), n * m.G * sizeof(double));
   if (sigma_s) std::memcpy(sigma_s, m.sigma_s.data(), n * m.G * sizeof(double));
   if (fleck) std::memcpy(fleck, m.fleck.data(), n * sizeof(double));

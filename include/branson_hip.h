@@ -269,6 +269,61 @@ int bh_adjoint_photons(bh_handle *h, const bh_photon *photons, uint64_t n,
                        const double *abs_adjoint, double *d_sigma_a, double *d_fleck,
                        bh_step_stats *stats);
 
+/* ---- Device-resident step physics ------------------------------------ */
+/* Per-cell material and initial state for step physics on the device: the
+ * cell's Region (src/region.h:27-76: opacity law op_a = opA + opB T^opC,
+ * scattering opS, heat capacity cv, density rho), its initial electron and
+ * radiation temperatures, the temperature of its SOURCE face and that face
+ * (Cell::get_source_face, src/cell.h:69-76; -1 for none).  n_cells entries
+ * each, for the cells of the last bh_set_mesh. */
+typedef struct bh_material {
+  const double *opA, *opB, *opC, *opS;
+  const double *cv, *rho;
+  const double *T_e, *T_r, *T_s;
+  const int8_t *source_face;
+} bh_material;
+
+/* What one resident step reports (IMC_State::print_conservation quantities,
+ * src/imc_state.h:174-289, less the pre-census energy after step 1, which is
+ * the previous step's post_census_E). */
+typedef struct bh_resident_report {
+  double total_E;         /* E_tot of the photon counts (all ranks)        */
+  double emission_E, source_E;
+  double pre_census_E;    /* step 1: the initial census energy             */
+  double pre_mat_E, post_mat_E, absorbed_E;
+  bh_step_stats stats;    /* transport and census of the step             */
+  uint64_t n_new;         /* new photons sourced this step                 */
+  int32_t etot_serial;    /* 1: E_tot was summed serially in cell order (more
+                             than one rank, or a photon count that a
+                             parallel sum's last bits could have changed);
+                             0: parallel sum, every count proven equal     */
+  int32_t pad;
+} bh_resident_report;
+
+/* Uploads the material (replaces any earlier one).  replicated_factor,
+ * n_ranks and rank are the replicated-mode share of this engine
+ * (Mesh::calculate_photon_energy, src/mesh.h:237-323). */
+int bh_set_material(bh_handle *h, const bh_material *m, double replicated_factor,
+                    uint32_t n_ranks, uint32_t rank);
+
+/* One replicated time step with its physics on the device
+ * (src/replicated_driver.h:55-101): Mesh::calculate_photon_energy
+ * (src/mesh.h:237-323, sums in cell order as the reference's loop), source
+ * (initial census at step 1), transport, census and Mesh::update_temperature
+ * (src/mesh.h:327-423).  Temperatures, opacities, energies and tallies stay
+ * on the device; only the report crosses to the host.  `step` is the
+ * reference's cycle number (1 = first step: initial census). */
+int bh_step_resident(bh_handle *h, uint64_t n_user_photons, uint32_t seed, uint32_t step,
+                     uint32_t rank, double dt, double next_dt, bh_resident_report *rep);
+
+/* Device-resident state after bh_step_resident: electron temperature and
+ * the diagnostic radiation temperature (T_r = (track_E / (V dt a c))^1/4),
+ * and the opacities and Fleck factor of the last step.  Any pointer may be
+ * NULL; n_cells entries each (sigma: n_cells * n_groups). */
+int bh_get_temperature(bh_handle *h, double *T_e, double *T_r);
+int bh_get_physics(bh_handle *h, double *sigma_a, double *sigma_s, double *fleck,
+                   double *emission_E, double *source_E);
+
 /* Counter-based RNG as the device computes it: n draws of RNG(seed, stream)
  * from counter ctr0 (src/RNG.h:262-342).  Runs on the device. */
 int bh_rng_draws(bh_handle *h, uint32_t seed, uint64_t stream, uint64_t ctr0,

@@ -32,6 +32,11 @@ typedef struct bh_driver_config {
   uint32_t n_ranks;      /* ranks sharing the source energy; 0 = 1            */
   uint64_t n_photons;    /* override of the deck's <photons>; 0 = deck value  */
   double t_stop;         /* override of the deck's <t_stop>; 0 = deck value   */
+  int32_t device_physics; /* 1: each step's physics (calculate_photon_energy,
+                             update_temperature) runs on the device and the
+                             material state stays there (bh_step_resident);
+                             0: host physics, per-step uploads             */
+  int32_t reserved;      /* set to 0                                          */
 } bh_driver_config;
 
 /* Per-step diagnostics: IMC_State::print_conservation's quantities
@@ -83,7 +88,9 @@ int bh_driver_get_tallies(const bh_driver *d, double *abs_E, double *track_E);
 int bh_driver_get_temperature(const bh_driver *d, double *T_e, double *T_r);
 int bh_driver_get_mesh(const bh_driver *d, double *nodes, int32_t *bc, uint32_t *next,
                        uint32_t *region);
-/* Cell data and source energies of the step last prepared. */
+/* Cell data and source energies of the step last prepared (with
+ * device_physics: of the step last transported; census_E and total_E are
+ * then not kept and come back as 0). */
 int bh_driver_get_physics(const bh_driver *d, double *sigma_a, double *sigma_s, double *fleck,
                           double *emission_E, double *source_E, double *census_E,
                           int8_t *source_face, double *total_E);
