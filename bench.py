@@ -102,6 +102,25 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
     return out
 
 
+def _run_resident(capi, steps, warmup, deck=DECK, n_photons=N_PHOTONS, n_groups=1):
+    """Whole steps with the step physics on the device (driver device_physics):
+    wall clock of prepare + transport + finish, nothing but the step report
+    crossing PCIe.  Histories per second of whole steps."""
+    reserve = n_photons * (1 + min(steps, 8))
+    kw = dict(algorithm=capi.ALGO_PERSISTENT, n_groups=n_groups, device_physics=True)
+    if warmup:
+        with capi.Driver(deck, t_stop=DT * warmup, **kw) as w:
+            w.run()
+    hist, secs = 0, 0.0
+    with capi.Driver(deck, t_stop=DT * steps, reserve_photons=reserve, **kw) as d:
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            r = d.step()
+            secs += time.perf_counter() - t0
+            hist += r["transported"]
+    return hist / secs
+
+
 def _roofline(form, run):
     """Average over the timed launches: algorithmic bytes per launch over the
     mean HIP-event duration of the transport kernel."""
@@ -193,6 +212,8 @@ def _bench_hohlraum(a, capi):
               "speedup_vs_plain_fp64": (ch["hist"] / ch["dev_s"]) / hist,
               "step1_tally_error_vs_fp64": _tally_error(plain["step1"], ch["step1"]),
               "replayed_histories": ch["replayed"]}
+    resident = _run_resident(capi, a.steps, a.warmup, deck=HOHLRAUM, n_photons=n_ph,
+                             n_groups=30)
     return _common(METRIC, hist, a.steps, a.warmup, 1e3 * plain["dev_s"] / a.steps, "f64",
                    {"workload": "hohlraum_64: 64^3 cells of 0.01 cm, 4-cell wall sigma_a=1000 "
                                 "rho=19, fill sigma_a=0.1 sigma_s=0.01 rho=0.001, 0.3 keV "
@@ -201,7 +222,9 @@ def _bench_hohlraum(a, capi):
                     "cell_records": "lattice" if plain["lattice"] else "general",
                     "tally": plain["tally"],
                     "histories": plain["hist"], "segments": plain["segs"]},
-                   segments_per_s=plain["segs"] / plain["dev_s"], chosen=chosen)
+                   segments_per_s=plain["segs"] / plain["dev_s"],
+                   whole_step_histories_per_s=plain["hist"] / plain["step_s"],
+                   resident_whole_step_histories_per_s=resident, chosen=chosen)
 
 
 def _bench_batched(a, capi):
@@ -368,6 +391,8 @@ def main(argv=None):
         # whole step: host physics, uploads of cells and source energies over
         # PCIe, device phase, tally copy-out and temperature update
         "whole_step_histories_per_s": plain["hist"] / plain["step_s"],
+        # whole step with the step physics on the device (no per-step uploads)
+        "resident_whole_step_histories_per_s": _run_resident(capi, a.steps, a.warmup),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": _traffic_from_profile(),

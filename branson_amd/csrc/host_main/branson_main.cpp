@@ -3,7 +3,7 @@
 // (src/main.cc:39-154, src/imc_state.h:264-284).
 //
 //   branson_hip <input.xml> [--fp32] [--persistent] [--groups N]
-//               [--photons N] [--device D]
+//               [--photons N] [--device D] [--device-physics]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -14,7 +14,7 @@
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::printf("Usage: branson_hip <path_to_input_file> [--fp32] [--persistent] "
-                "[--groups N] [--photons N] [--device D]\n");
+                "[--groups N] [--photons N] [--device D] [--device-physics]\n");
     return EXIT_FAILURE;
   }
   bh_driver_config cfg;
@@ -25,6 +25,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--groups") && i + 1 < argc) cfg.n_groups = (uint32_t)std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--photons") && i + 1 < argc) cfg.n_photons = std::strtoull(argv[++i], nullptr, 10);
     else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) cfg.device = std::atoi(argv[++i]);
+    else if (!std::strcmp(argv[i], "--device-physics")) cfg.device_physics = 1;
     else {
       std::fprintf(stderr, "unknown argument %s\n", argv[i]);
       return EXIT_FAILURE;
