@@ -36,7 +36,7 @@ EXPORTS = [
     "bh_abi_version", "bh_create", "bh_destroy", "bh_last_error", "bh_sync", "bh_set_mesh",
     "bh_set_source_energy", "bh_source", "bh_source_from_energy", "bh_set_instances",
     "bh_source_instances", "bh_transport", "bh_set_material", "bh_step_resident",
-    "bh_get_temperature", "bh_get_physics",
+    "bh_get_temperature", "bh_get_physics", "bh_comb_census",
     "bh_step_graph", "bh_get_tallies", "bh_census_size", "bh_census_export",
     "bh_photons_export",
     "bh_census_import",
@@ -94,6 +94,15 @@ class ResidentReport(C.Structure):
         return d
 
 
+class CombReport(C.Structure):
+    _fields_ = [("n_before", C.c_uint64), ("n_after", C.c_uint64), ("comb_E", C.c_double),
+                ("census_E_before", C.c_double), ("census_E_after", C.c_double),
+                ("comb_serial", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 _lib = None
 
 
@@ -135,6 +144,7 @@ def lib() -> C.CDLL:
     L.bh_step_resident.argtypes = [P, U64, U32, U32, U32, D, D, P]
     L.bh_get_temperature.argtypes = [P, P, P]
     L.bh_get_physics.argtypes = [P, P, P, P, P, P]
+    L.bh_comb_census.argtypes = [P, C.c_int64, U32, U64, U64, P]
     for name in EXPORTS:
         if name not in ("bh_destroy", "bh_last_error", "bh_abi_version"):
             getattr(L, name).restype = I
@@ -317,6 +327,14 @@ class Engine:
         self._check(lib().bh_adjoint_photons(self._h, _p(photons), len(photons), _p(g), _p(da),
                                              _p(df), C.byref(st)))
         return da, df, st.as_dict()
+
+    def comb_census(self, max_census: int, seed: int, stream: int, ctr0: int = 0) -> dict:
+        """bh_comb_census: comb_photons (src/census_functions.h:48-93) on the
+        resident census, draws from RNG(seed, stream) at counter ctr0."""
+        rep = CombReport()
+        self._check(lib().bh_comb_census(self._h, max_census, seed, stream, ctr0,
+                                         C.byref(rep)))
+        return rep.as_dict()
 
     # -- device-resident step physics -----------------------------------------
     def set_material(self, opA, opB, opC, opS, cv, rho, T_e, T_r, T_s, source_face,

@@ -14,7 +14,10 @@
 #include "../../include/branson_hip.h"
 #include "bh_kernels.h"
 #include "bh_physics.h"
+#include "bh_comb.h"
 #include "bh_tracking.h"
+
+#include <hipcub/hipcub.hpp>
 
 using namespace bh;
 
@@ -106,6 +109,11 @@ struct bh_handle {
   PhysTotals* d_phys = nullptr;
   SourceInst* d_inst2 = nullptr;  // [census, new] source records, E_tot set on the device
   DArr<double> sum_part;          // block partials of the deterministic sums
+  // comb (bh_comb_census): sort buffers, keep flags, per-cell energies
+  DArr<uint32_t> comb_keys, comb_idx, comb_sorted;
+  DArr<uint8_t> comb_keep, comb_tmp;
+  DArr<double> comb_cellE;
+  DArr<unsigned long long> comb_ctr;
   uint64_t serial_etot_steps = 0; // steps whose E_tot needed the serial sum
   uint32_t dims[3] = {0, 0, 0};
   DArr<CellRec> rec;
@@ -619,6 +627,13 @@ void bh_destroy(bh_handle* h) {
   release(h->mat_e);
   release(h->tr_diag);
   release(h->sum_part);
+  release(h->comb_keys);
+  release(h->comb_idx);
+  release(h->comb_sorted);
+  release(h->comb_keep);
+  release(h->comb_tmp);
+  release(h->comb_cellE);
+  release(h->comb_ctr);
   void* ptrs[] = {h->W.slab, h->C.slab, h->d_total, h->d_ctot, h->d_ctrs, h->d_queue,
                   h->d_phys, h->d_inst2};
   for (void* p : ptrs)
@@ -1414,6 +1429,133 @@ int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
   if (ctrs.stuck)
     return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
                 ctrs.stuck, kMaxSegments);
+  return BH_OK;
+}
+
+// ---- census population control ------------------------------------------------
+int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint64_t stream,
+                   uint64_t ctr0, bh_comb_report* rep) {
+  if (!h || !h->stream) return BH_ERR_ARG;
+  if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_comb_census before bh_set_mesh");
+  if (max_census_photons <= 0) return fail(h, BH_ERR_ARG, "max_census_photons must be > 0");
+  if (h->n_new) return fail(h, BH_ERR_STATE, "new photons are waiting for transport");
+  if (h->n_cen >= 0xFFFFFFFFull) return fail(h, BH_ERR_ARG, "census too large to comb");
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  const uint32_t n = (uint32_t)h->n_cen, nc = h->n_cells;
+  if (rep) rep->n_before = n;
+  if (n == 0) return BH_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  const uint64_t ctr_hi = (uint64_t)seed << 32;
+  // E_census: deterministic parallel sum, with the bound on the serial loop's
+  const uint32_t nb = std::max<uint32_t>(1u, std::min<uint32_t>(kSumMaxBlocks, div_up(n, 4096)));
+  TRY(ensure(h, h->sum_part, (size_t)8 * nb));
+  if (!h->d_phys) HIPCHK(h, hipMalloc(&h->d_phys, sizeof(PhysTotals)));
+  k_energy_partials<<<nb, kSumNT, 0, h->stream>>>(h->W.soa.E, n, h->sum_part.p);
+  k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->global_E);
+  HIPCHK(h, hipGetLastError());
+  char* hp = static_cast<char*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp + 3072, &h->d_phys->global_E, sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+  // stable sort of the census by cell: each cell's photons in census order
+  TRY(ensure(h, h->comb_keys, n));
+  TRY(ensure(h, h->comb_idx, n));
+  TRY(ensure(h, h->comb_sorted, n));
+  TRY(ensure(h, h->comb_keep, n));
+  TRY(ensure(h, h->comb_cellE, nc));
+  TRY(ensure(h, h->comb_ctr, 2));
+  TRY(ensure(h, h->counts, nc));
+  TRY(ensure(h, h->off, nc));
+  k_iota_u32<<<grid_for(n, 256), 256, 0, h->stream>>>(h->comb_idx.p, n);
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < nc) ++bits;
+  size_t tmp_bytes = 0;
+  HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, h->W.soa.cell, h->comb_keys.p,
+                                               h->comb_idx.p, h->comb_sorted.p, (int)n, 0, bits,
+                                               h->stream));
+  TRY(ensure(h, h->comb_tmp, tmp_bytes));
+  HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->comb_tmp.p, tmp_bytes, h->W.soa.cell,
+                                               h->comb_keys.p, h->comb_idx.p, h->comb_sorted.p,
+                                               (int)n, 0, bits, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->counts.p, 0, nc * sizeof(uint64_t), h->stream));
+  k_cell_hist<<<grid_for(n, 256), 256, 0, h->stream>>>(
+      h->W.soa.cell, n, reinterpret_cast<unsigned long long*>(h->counts.p));
+  HIPCHK(h, hipGetLastError());
+  TRY(scan_u64(h, h->counts.p, nc, h->off.p, 0));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double E_par;
+  std::memcpy(&E_par, hp + 3072, sizeof E_par);
+  const double delta = 2.0 * ((double)n + 64.0) * 0x1p-53 * E_par;
+  const double m = (double)max_census_photons;
+  CombParams cp{E_par / m, (E_par - delta) / m, (E_par + delta) / m, ctr0, stream, ctr_hi};
+  auto comb = [&](uint32_t* amb, uint64_t* kept) -> int {
+    // comb_ctr: [0] ambiguous flag (low 32 bits), [1] survivors
+    HIPCHK(h, hipMemsetAsync(h->comb_ctr.p, 0, 2 * sizeof(unsigned long long), h->stream));
+    k_comb_cells<<<grid_for(nc, 256), 256, 0, h->stream>>>(
+        h->comb_sorted.p, reinterpret_cast<const unsigned long long*>(h->off.p),
+        reinterpret_cast<const unsigned long long*>(h->counts.p), nc, h->W.soa.E, cp,
+        h->comb_keep.p, h->comb_cellE.p, reinterpret_cast<uint32_t*>(h->comb_ctr.p),
+        h->comb_ctr.p + 1);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(hp + 3072, h->comb_ctr.p, 2 * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    uint64_t v[2];
+    std::memcpy(v, hp + 3072, sizeof v);
+    *amb = (uint32_t)v[0];
+    *kept = v[1];
+    return BH_OK;
+  };
+  uint32_t amb = 0;
+  uint64_t kept = 0;
+  TRY(comb(&amb, &kept));
+  const bool serial = amb != 0;
+  if (serial) {  // the reference's serial census energy, then comb again
+    k_serial_energy<<<1, kWave, 0, h->stream>>>(h->W.soa.E, n, &h->d_phys->global_E);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(hp + 3072, &h->d_phys->global_E, sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double E_ser;
+    std::memcpy(&E_ser, hp + 3072, sizeof E_ser);
+    cp.comb_E = cp.comb_lo = cp.comb_hi = E_ser / m;
+    E_par = E_ser;
+    TRY(comb(&amb, &kept));
+  }
+  k_comb_commit<<<grid_for(n, 256), 256, 0, h->stream>>>(h->W.soa, n, h->comb_keep.p,
+                                                         h->comb_cellE.p);
+  HIPCHK(h, hipGetLastError());
+  // survivors, in census order, into C (the census compaction, each photon
+  // keeping its life)
+  const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kCensusTile), 1);
+  TRY(ensure(h, h->tiles2, n_tiles));
+  TRY(ensure(h, h->parts, n_tiles));
+  TRY(grow_store(h, h->C, n, 0));
+  const unsigned cgrid = (unsigned)std::min<uint64_t>(n_tiles, 8192);
+  k_census_count<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, 0, n, h->tiles2.p, h->parts.p,
+                                                     nullptr);
+  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, n_tiles, nullptr, h->d_ctot);
+  TRY(ensure(h, h->off, std::max<uint64_t>(n_tiles, nc)));
+  TRY(scan_u64(h, h->tiles2.p, n_tiles, h->off.p, 2));
+  k_census_scatter<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, 0, n, h->off.p, h->C.soa,
+                                                       -1.0, nullptr);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  CensusPartial tot;
+  std::memcpy(&tot, hp, sizeof tot);
+  if (tot.n_census != kept)
+    return fail(h, BH_ERR_DEVICE, "comb kept %llu photons, compaction found %llu",
+                (unsigned long long)kept, (unsigned long long)tot.n_census);
+  std::swap(h->W, h->C);
+  h->n_cen = kept;
+  if (rep) {
+    rep->n_after = kept;
+    rep->comb_E = cp.comb_E;
+    rep->census_E_before = E_par;
+    rep->census_E_after = tot.census_E;
+    rep->comb_serial = serial ? 1 : 0;
+  }
   return BH_OK;
 }
 

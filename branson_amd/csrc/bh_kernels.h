@@ -775,7 +775,8 @@ __device__ __forceinline__ void census_scatter_tile(const PhotonSoA& W, uint64_t
       C.w[o] = W.w[i];
       C.E[o] = W.E[i];
       C.E0[o] = W.E0[i];
-      C.life[o] = census_life;
+      // census_life < 0: survivors of a comb keep their own life
+      C.life[o] = census_life >= 0.0 ? census_life : W.life[i];
       C.ctr[o] = W.ctr[i];
       C.stream[o] = W.stream[i];
       C.cell[o] = W.cell[i];

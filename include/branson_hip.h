@@ -269,6 +269,28 @@ int bh_adjoint_photons(bh_handle *h, const bh_photon *photons, uint64_t n,
                        const double *abs_adjoint, double *d_sigma_a, double *d_fleck,
                        bh_step_stats *stats);
 
+/* ---- Census population control ---------------------------------------- */
+typedef struct bh_comb_report {
+  uint64_t n_before, n_after;  /* census photons before and after          */
+  double comb_E;               /* E_census / max_census_photons            */
+  double census_E_before, census_E_after;
+  int32_t comb_serial;         /* 1: E_census summed serially (a decision
+                                  depended on its last bits)               */
+  int32_t pad;
+} bh_comb_report;
+
+/* comb_photons (src/census_functions.h:48-93) on the resident census: each
+ * photon survives with probability E / comb_E (comb_E = E_census /
+ * max_census_photons), survivors take energy comb_E, a cell always keeps its
+ * last photon when the others are combed away, and each cell's survivors
+ * share the cell's energy correction, so every cell's census energy is
+ * conserved.  The reference passes an RNG*; here the draws are those of
+ * RNG(seed, stream) from counter ctr0, one per census photon in census
+ * order.  Survivors keep their census order.  (The reference defines the
+ * function but never calls it.) */
+int bh_comb_census(bh_handle *h, int64_t max_census_photons, uint32_t seed, uint64_t stream,
+                   uint64_t ctr0, bh_comb_report *rep);
+
 /* ---- Device-resident step physics ------------------------------------ */
 /* Per-cell material and initial state for step physics on the device: the
  * cell's Region (src/region.h:27-76: opacity law op_a = opA + opB T^opC,

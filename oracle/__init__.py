@@ -163,6 +163,8 @@ def lib() -> C.CDLL:
         L.orc_dos_transport.restype = U64
         L.orc_post_process.argtypes = [D, P, U64, P, U64, P]
         L.orc_post_process.restype = U64
+        L.orc_comb_photons.argtypes = [P, U64, C.c_uint32, C.c_int64, C.c_uint32, U64, U64]
+        L.orc_comb_photons.restype = U64
         L.orc_sim_create.argtypes = [P]
         L.orc_sim_create.restype = P
         L.orc_sim_destroy.argtypes = [P]
@@ -351,6 +353,15 @@ def post_process(next_dt: float, photons: np.ndarray):
         "census_E": st.census_E, "exit_E": st.exit_E, "n_census": st.n_census,
         "n_exit": st.n_exit, "n_killed": st.n_killed,
     }
+
+
+def comb_photons(photons: np.ndarray, n_cells: int, max_census: int, seed: int, stream: int,
+                 ctr0: int = 0) -> np.ndarray:
+    """comb_photons (src/census_functions.h:48-93) on a census in order, with
+    RNG(seed, stream) from counter ctr0.  Returns the survivors in order."""
+    ph = np.ascontiguousarray(photons, dtype=PHOTON_DTYPE).copy()
+    k = lib().orc_comb_photons(_ptr(ph), len(ph), n_cells, max_census, seed, stream, ctr0)
+    return ph[:k].copy()
 
 
 # --------------------------------------------------------------------------

@@ -655,6 +655,47 @@ uint64_t orc_dos_transport(const orc_cells *cells, orc_photon *photons,
 
 /* post_process_photons (src/post_process_functions.h:32-59): census keeps
  * transport order and gets life_dx = c * next_dt. */
+/* comb_photons (src/census_functions.h:48-93).  The reference's maps keyed
+ * by cell are arrays here; the loops and their order are the reference's. */
+uint64_t orc_comb_photons(orc_photon *photons, uint64_t n, uint32_t n_cells,
+                          int64_t max_census_photons, uint32_t seed, uint64_t stream,
+                          uint64_t ctr0) {
+  uint32_t *count = (uint32_t *)calloc(n_cells ? n_cells : 1, sizeof(uint32_t));
+  double *cell_E = (double *)calloc(n_cells ? n_cells : 1, sizeof(double));
+  double *corrected = (double *)calloc(n_cells ? n_cells : 1, sizeof(double));
+  double census_total_E = 0.0; /* get_photon_list_E (:40-46) */
+  for (uint64_t i = 0; i < n; ++i) census_total_E += photons[i].E;
+  const double comb_E = census_total_E / (double)max_census_photons; /* :56-58 */
+  for (uint64_t i = 0; i < n; ++i) {                                  /* :60-63 */
+    count[photons[i].cell]++;
+    cell_E[photons[i].cell] += photons[i].E;
+  }
+  uint64_t rng[4];
+  orc_rng_init(rng, seed, stream);
+  rng[0] = ctr0;
+  uint64_t kept = 0;
+  for (uint64_t i = 0; i < n; ++i) { /* :65-79 */
+    const uint32_t c = photons[i].cell;
+    const double p_kill = 1.0 - photons[i].E / comb_E;
+    const double rand_check = orc_rng_next(rng);
+    if (rand_check > p_kill || count[c] == 1) {
+      photons[i].E = comb_E;
+      corrected[c] += comb_E;
+      photons[kept++] = photons[i];
+    } else {
+      count[c]--;
+    }
+  }
+  for (uint64_t j = 0; j < kept; ++j) { /* :81-90 */
+    const uint32_t c = photons[j].cell;
+    photons[j].E = photons[j].E + (cell_E[c] - corrected[c]) / (double)count[c];
+  }
+  free(count);
+  free(cell_E);
+  free(corrected);
+  return kept;
+}
+
 uint64_t orc_post_process(double next_dt, orc_photon *photons, uint64_t n,
                           orc_photon *census_out, uint64_t cap,
                           orc_census_stats *st) {

@@ -138,6 +138,16 @@ typedef struct orc_census_stats {
   uint64_t n_exit;
   uint64_t n_killed;
 } orc_census_stats;
+/* comb_photons (src/census_functions.h:48-93), one rank: combs the census
+ * (in census order) down toward max_census_photons photons of energy
+ * E_census / max_census_photons, with per-cell energy correction.  The
+ * reference takes an RNG*; here it is RNG(seed, stream) from counter ctr0,
+ * one draw per census photon in order.  Survivors are written to the front
+ * of `photons` in census order; returns their number. */
+uint64_t orc_comb_photons(orc_photon *photons, uint64_t n, uint32_t n_cells,
+                          int64_t max_census_photons, uint32_t seed, uint64_t stream,
+                          uint64_t ctr0);
+
 uint64_t orc_post_process(double next_dt, orc_photon *photons, uint64_t n,
                           orc_photon *census_out, uint64_t cap,
                           orc_census_stats *stats);
