@@ -344,8 +344,6 @@ __device__ __forceinline__ bool boundary_event(const DevCells& cells, const Cell
   return false;
 }
 
-// Draw states of a segment.
-enum : int { kDrawMu = 0, kDrawPhi, kDrawTest, kDrawGroup, kDrawDistance, kDrawDone };
 
 // ---- Wave-cooperative draws ------------------------------------------------
 // A segment takes one draw (the distance), or five after an effective
@@ -418,6 +416,9 @@ __device__ __forceinline__ void wave_draws(bool want, const Live<R>& L, WaveDraw
   wave_lds_sync();
 }
 
+// Draw states of the lane-by-lane form of a segment's draws.
+enum : int { kDrawMu = 0, kDrawPhi, kDrawTest, kDrawGroup, kDrawDistance, kDrawDone };
+
 // What one FP64 segment tells the adjoint (filled when kInfo is set).
 struct SegInfo {
   uint32_t c;            // cell of the segment
@@ -454,53 +455,97 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
   double2 sg = cv.sigma(L.group);
   double total_sigma_s = (1.0 - f) * sg.x + sg.y;
 
+  // Draws in the reference's order: after an effective scatter the new
+  // direction (mu, phi), the re-emission test and, when it re-samples, the
+  // group; then every photon's distance to scatter.  With the wave's draws
+  // the code is straight-line, so the distance draw and its log run once for
+  // the whole wave; lane by lane (replay, adjoint) it is a loop around one
+  // Threefry site, which keeps those kernels small.
   double d_s = 1.0e100;
-  int st = L.pending ? kDrawMu : kDrawDistance;
-  if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
-  double mu = 0.0;
-  while (st != kDrawDone) {
-    // drawn by the wave beforehand, or here (the segment's only Threefry site)
-    const double r = dr ? dr->take(L.rng) : L.rng.next();
-    if (st == kDrawMu) {  // get_uniform_angle (src/sampling_functions.h:56-71)
-      mu = r * 2.0 - 1.0;
-      st = kDrawPhi;
-    } else if (st == kDrawPhi) {
-      const double phi = r * 2.0 * kPi;
+  if (dr) {  // the wave drew them beforehand
+    auto draw = [&]() -> double { return dr->take(L.rng); };
+    if (L.pending) {  // src/history_based_transport.h:95-100
+      // get_uniform_angle (src/sampling_functions.h:56-71)
+      const double mu = draw() * 2.0 - 1.0;
+      const double phi = draw() * 2.0 * kPi;
       const double sin_theta = sqrt(1.0 - mu * mu);
       double sp, cp;
       sincos(phi, &sp, &cp);
       L.u = sin_theta * cp;
       L.v = sin_theta * sp;
       L.w = mu;
-      st = kDrawTest;
-    } else if (st == kDrawTest) {  // re-emission test (:97)
-      const bool resample = r > (sg.y / total_sigma_s);
+      const bool resample = draw() > (sg.y / total_sigma_s);  // re-emission test (:97)
       if constexpr (kInfo) {
         si->reemit = resample ? 2 : 1;
         si->r_sa = sg.x;
         si->r_f = f;
         si->r_sig = total_sigma_s;
       }
-      st = resample ? kDrawGroup : kDrawDistance;
-    } else if (st == kDrawGroup) {  // sample_emission_group walk (:98)
-      double cdf = r;
-      int g = -1;
-      const double norm = 1.0 / (cv.sigma(0).x * (double)G);
-      while (cdf > 0) {
-        g++;
-        if ((uint32_t)g >= G) {
-          g = (int)G - 1;
-          break;
+      if (resample) {  // sample_emission_group walk (:98)
+        double cdf = draw();
+        int g = -1;
+        const double norm = 1.0 / (cv.sigma(0).x * (double)G);
+        while (cdf > 0) {
+          g++;
+          if ((uint32_t)g >= G) {
+            g = (int)G - 1;
+            break;
+          }
+          cdf -= cv.sigma((uint32_t)g).x * norm;
         }
-        cdf -= cv.sigma((uint32_t)g).x * norm;
+        L.group = (uint32_t)g;
+        sg = cv.sigma((uint32_t)g);
+        total_sigma_s = (1.0 - f) * sg.x + sg.y;
       }
-      L.group = (uint32_t)g;
-      sg = cv.sigma((uint32_t)g);
-      total_sigma_s = (1.0 - f) * sg.x + sg.y;
-      st = (total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
-    } else {  // distance to scatter (:62-63)
-      d_s = -log(r) / total_sigma_s;
-      st = kDrawDone;
+    }
+    if (total_sigma_s > 0.0) d_s = -log(draw()) / total_sigma_s;  // distance to scatter (:62-63)
+  } else {  // lane by lane: one Threefry site, a draw per loop trip
+    int st = L.pending ? kDrawMu : kDrawDistance;
+    if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
+    double mu = 0.0;
+    while (st != kDrawDone) {
+      const double r = L.rng.next();  // the loop's only Threefry site
+      if (st == kDrawMu) {  // get_uniform_angle (src/sampling_functions.h:56-71)
+        mu = r * 2.0 - 1.0;
+        st = kDrawPhi;
+      } else if (st == kDrawPhi) {
+        const double phi = r * 2.0 * kPi;
+        const double sin_theta = sqrt(1.0 - mu * mu);
+        double sp, cp;
+        sincos(phi, &sp, &cp);
+        L.u = sin_theta * cp;
+        L.v = sin_theta * sp;
+        L.w = mu;
+        st = kDrawTest;
+      } else if (st == kDrawTest) {  // re-emission test (:97)
+        const bool resample = r > (sg.y / total_sigma_s);
+        if constexpr (kInfo) {
+          si->reemit = resample ? 2 : 1;
+          si->r_sa = sg.x;
+          si->r_f = f;
+          si->r_sig = total_sigma_s;
+        }
+        st = resample ? kDrawGroup : kDrawDistance;
+      } else if (st == kDrawGroup) {  // sample_emission_group walk (:98)
+        double cdf = r;
+        int g = -1;
+        const double norm = 1.0 / (cv.sigma(0).x * (double)G);
+        while (cdf > 0) {
+          g++;
+          if ((uint32_t)g >= G) {
+            g = (int)G - 1;
+            break;
+          }
+          cdf -= cv.sigma((uint32_t)g).x * norm;
+        }
+        L.group = (uint32_t)g;
+        sg = cv.sigma((uint32_t)g);
+        total_sigma_s = (1.0 - f) * sg.x + sg.y;
+        st = (total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
+      } else {  // distance to scatter (:62-63)
+        d_s = -log(r) / total_sigma_s;
+        st = kDrawDone;
+      }
     }
   }
   L.pending = false;
@@ -605,16 +650,13 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
   double2 sg = cv.sigma(L.group);
   double total_sigma_s = (1.0 - f) * sg.x + sg.y;
 
+  // the FP64 form's draws, in its order (straight-line: the distance draw
+  // and its logf run once for the whole wave)
   float d_s = 1.0e30f, e_s = 0.0f;
-  int st = L.pending ? kDrawMu : kDrawDistance;
-  if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
-  double mu = 0.0;
-  while (st != kDrawDone) {
-    const double r = dr ? dr->take(L.rng) : L.rng.next();
-    if (st == kDrawMu) {
-      mu = r * 2.0 - 1.0;
-      st = kDrawPhi;
-    } else if (st == kDrawPhi) {
+  if (dr) {  // the wave drew them beforehand
+    auto draw = [&]() -> double { return dr->take(L.rng); };
+    if (L.pending) {
+      const double mu = draw() * 2.0 - 1.0;
       // sin(theta) from the FP64 cosine: in FP32, mu can round to +-1 and
       // leave a direction with two exactly-zero components, whose face
       // distances are -inf.  In FP64 sqrt(1 - mu^2) >= 2^-25.5 is never 0.
@@ -622,7 +664,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
       // (exact in FP64): phi = q pi/2 + f pi/2, so the FP32 argument of
       // sincospif is below pi/4 and rounds by at most 2^-25, and the quarter
       // turns are exact swaps and sign changes
-      const double r4 = r * 4.0;
+      const double r4 = draw() * 4.0;
       const double q = rint(r4);
       float s0, c0;
       sincospif(0.5f * (float)(r4 - q), &s0, &c0);
@@ -636,32 +678,88 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, F
       const float e = L.eP + L.eT;  // |e| seen from the new direction
       L.eP = e;
       L.eT = e;
-      st = kDrawTest;
-    } else if (st == kDrawTest) {
-      st = (r > (sg.y / total_sigma_s)) ? kDrawGroup : kDrawDistance;
-    } else if (st == kDrawGroup) {
-      double cdf = r;
-      int g = -1;
-      const double norm = 1.0 / (cv.sigma(0).x * (double)G);
-      while (cdf > 0) {
-        g++;
-        if ((uint32_t)g >= G) {
-          g = (int)G - 1;
-          break;
+      if (draw() > (sg.y / total_sigma_s)) {  // re-emission: new group
+        double cdf = draw();
+        int g = -1;
+        const double norm = 1.0 / (cv.sigma(0).x * (double)G);
+        while (cdf > 0) {
+          g++;
+          if ((uint32_t)g >= G) {
+            g = (int)G - 1;
+            break;
+          }
+          cdf -= cv.sigma((uint32_t)g).x * norm;
         }
-        cdf -= cv.sigma((uint32_t)g).x * norm;
+        L.group = (uint32_t)g;
+        sg = cv.sigma((uint32_t)g);
+        total_sigma_s = (1.0 - f) * sg.x + sg.y;
       }
-      L.group = (uint32_t)g;
-      sg = cv.sigma((uint32_t)g);
-      total_sigma_s = (1.0 - f) * sg.x + sg.y;
-      st = (total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
-    } else {
-      float uf = (float)r;
+    }
+    if (total_sigma_s > 0.0) {
+      float uf = (float)draw();
       uf = (uf < 1.0f) ? uf : 0x1.fffffep-1f;  // (0,1) stays open after rounding
       const float sig = (float)total_sigma_s;
       d_s = -logf(uf) / sig;
       e_s = kE * d_s + 2.0f * kU / sig;
-      st = kDrawDone;
+    }
+  } else {  // lane by lane: one Threefry site, a draw per loop trip
+    int st = L.pending ? kDrawMu : kDrawDistance;
+    if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
+    double mu = 0.0;
+    while (st != kDrawDone) {
+      const double r = L.rng.next();  // the loop's only Threefry site
+      if (st == kDrawMu) {
+        mu = r * 2.0 - 1.0;
+        st = kDrawPhi;
+      } else if (st == kDrawPhi) {
+        // sin(theta) from the FP64 cosine: in FP32, mu can round to +-1 and
+        // leave a direction with two exactly-zero components, whose face
+        // distances are -inf.  In FP64 sqrt(1 - mu^2) >= 2^-25.5 is never 0.
+        // phi = 2 pi r, split as 4r = q + f with q = rint(4r) and |f| <= 1/2
+        // (exact in FP64): phi = q pi/2 + f pi/2, so the FP32 argument of
+        // sincospif is below pi/4 and rounds by at most 2^-25, and the quarter
+        // turns are exact swaps and sign changes
+        const double r4 = r * 4.0;
+        const double q = rint(r4);
+        float s0, c0;
+        sincospif(0.5f * (float)(r4 - q), &s0, &c0);
+        const int qi = (int)q & 3;
+        const float cp = (qi == 0) ? c0 : (qi == 1) ? -s0 : (qi == 2) ? -c0 : s0;
+        const float sp = (qi == 0) ? s0 : (qi == 1) ? c0 : (qi == 2) ? -s0 : -c0;
+        const float s_t = (float)sqrt(1.0 - mu * mu);
+        L.u = s_t * cp;
+        L.v = s_t * sp;
+        L.w = (float)mu;
+        const float e = L.eP + L.eT;  // |e| seen from the new direction
+        L.eP = e;
+        L.eT = e;
+        st = kDrawTest;
+      } else if (st == kDrawTest) {
+        st = (r > (sg.y / total_sigma_s)) ? kDrawGroup : kDrawDistance;
+      } else if (st == kDrawGroup) {
+        double cdf = r;
+        int g = -1;
+        const double norm = 1.0 / (cv.sigma(0).x * (double)G);
+        while (cdf > 0) {
+          g++;
+          if ((uint32_t)g >= G) {
+            g = (int)G - 1;
+            break;
+          }
+          cdf -= cv.sigma((uint32_t)g).x * norm;
+        }
+        L.group = (uint32_t)g;
+        sg = cv.sigma((uint32_t)g);
+        total_sigma_s = (1.0 - f) * sg.x + sg.y;
+        st = (total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
+      } else {
+        float uf = (float)r;
+        uf = (uf < 1.0f) ? uf : 0x1.fffffep-1f;  // (0,1) stays open after rounding
+        const float sig = (float)total_sigma_s;
+        d_s = -logf(uf) / sig;
+        e_s = kE * d_s + 2.0f * kU / sig;
+        st = kDrawDone;
+      }
     }
   }
   L.pending = false;
