@@ -43,18 +43,28 @@ __global__ __launch_bounds__(kSumNT) void k_energy_partials(const double* __rest
 }
 
 // The reference's get_photon_list_E (src/census_functions.h:40-46): one
-// serial chain over the census in order (uniform loop, scalar loads).
+// serial chain over the census in census order (perm: rank -> slot;
+// uniform loop, scalar loads).
 __global__ __launch_bounds__(kWave) void k_serial_energy(const double* __restrict__ E,
+                                                         const uint32_t* __restrict__ perm,
                                                          uint32_t n, double* __restrict__ out) {
   double s = 0.0;
 #pragma unroll 8
-  for (uint32_t i = 0; i < n; ++i) s += E[i];
+  for (uint32_t j = 0; j < n; ++j) s += E[perm[j]];
   if (threadIdx.x == 0) *out = s;
 }
 
 __global__ void k_iota_u32(uint32_t* __restrict__ out, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = i;
+}
+
+// Sort keys in census order: the cell of the photon of rank j.
+__global__ void k_cells_by_rank(const uint32_t* __restrict__ cell,
+                                const uint32_t* __restrict__ perm, uint32_t n,
+                                uint32_t* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = cell[perm[j]];
 }
 
 __global__ void k_cell_hist(const uint32_t* __restrict__ cell, uint32_t n,
@@ -67,7 +77,10 @@ __global__ void k_cell_hist(const uint32_t* __restrict__ cell, uint32_t n,
 // census order) combed as the reference's loop combs them.  keep[i] and the
 // cell's corrected survivor energy are written; ambiguous is set when a
 // decision differs at the two ends of comb_E's bound.
+// sorted: census ranks grouped by cell, each cell's in census order; perm:
+// rank -> slot of W; keep is indexed by rank.
 __global__ __launch_bounds__(256) void k_comb_cells(const uint32_t* __restrict__ sorted,
+                                                    const uint32_t* __restrict__ perm,
                                                     const unsigned long long* __restrict__ off,
                                                     const unsigned long long* __restrict__ cnt,
                                                     uint32_t n_cells,
@@ -81,13 +94,13 @@ __global__ __launch_bounds__(256) void k_comb_cells(const uint32_t* __restrict__
   const uint64_t m = cnt[c], base = off[c];
   if (m == 0) return;
   double cell_E = 0.0;
-  for (uint64_t j = 0; j < m; ++j) cell_E += E[sorted[base + j]];
+  for (uint64_t j = 0; j < m; ++j) cell_E += E[perm[sorted[base + j]]];
   uint32_t kept = 0;
   double corrected = 0.0;
   bool amb = false;
   for (uint64_t j = 0; j < m; ++j) {
-    const uint32_t i = sorted[base + j];
-    const double e = E[i];
+    const uint32_t i = sorted[base + j];  // census rank
+    const double e = E[perm[i]];
     const double p_kill = 1.0 - e / cp.comb_E;
     uint64_t o0, o1;
     Threefry::block(cp.ctr0 + i, cp.ctr_hi, cp.stream, 0ull, o0, o1);
@@ -106,18 +119,24 @@ __global__ __launch_bounds__(256) void k_comb_cells(const uint32_t* __restrict__
   if (amb) atomicOr(ambiguous, 1u);
 }
 
-// Survivors take their cell's corrected energy and stay CENSUS; the rest
-// become KILLED, so the census compaction drops them.
-__global__ void k_comb_commit(PhotonSoA W, uint32_t n, const uint8_t* __restrict__ keep,
-                              const double* __restrict__ cell_E_new) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (keep[i]) {
-    W.E[i] = cell_E_new[W.cell[i]];
-    W.desc[i] = kCensus;
-  } else {
-    W.desc[i] = kKilled;
-  }
+// keep[j] as u64 flags for the survivors' exclusive scan.
+__global__ void k_keep_flags(const uint8_t* __restrict__ keep, uint32_t n,
+                             uint64_t* __restrict__ flag) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) flag[j] = keep[j];
+}
+
+// Survivors, in census order, to C[pos[j]] with their cell's corrected
+// energy and their new census rank as key (life and the rest unchanged).
+__global__ void k_comb_compact(PhotonSoA W, const uint32_t* __restrict__ perm, uint32_t n,
+                               const uint8_t* __restrict__ keep,
+                               const uint64_t* __restrict__ pos,
+                               const double* __restrict__ cell_E_new, PhotonSoA C) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || !keep[j]) return;
+  const uint32_t s = perm[j];
+  const uint64_t o = pos[j];
+  copy_photon(W, s, C, o, cell_E_new[W.cell[s]], (uint32_t)o);
 }
 
 }  // namespace bh

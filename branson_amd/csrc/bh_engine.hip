@@ -28,7 +28,8 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 inline uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Field sizes of PhotonSoA, in declaration order.
-constexpr size_t kFieldBytes[15] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 4, 4, 1, 1};
+constexpr int kFields = 16;
+constexpr size_t kFieldBytes[kFields] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 4, 4, 1, 1, 4};
 
 // One slab of device memory holding every field of `cap` photons.
 struct PhotonStore {
@@ -43,8 +44,8 @@ struct PhotonStore {
   }
   static PhotonSoA carve(void* slab, uint64_t cap) {
     char* p = static_cast<char*>(slab);
-    void* f[15];
-    for (int k = 0; k < 15; ++k) {
+    void* f[kFields];
+    for (int k = 0; k < kFields; ++k) {
       f[k] = p;
       p += align_up(kFieldBytes[k] * cap);
     }
@@ -64,11 +65,12 @@ struct PhotonStore {
     s.group = (uint32_t*)f[12];
     s.desc = (uint8_t*)f[13];
     s.stype = (uint8_t*)f[14];
+    s.key = (uint32_t*)f[15];
     return s;
   }
   static void* field(const PhotonSoA& s, int k) {
-    void* f[15] = {s.x, s.y, s.z, s.u, s.v, s.w, s.E, s.E0, s.life,
-                   s.ctr, s.stream, s.cell, s.group, s.desc, s.stype};
+    void* f[kFields] = {s.x, s.y, s.z, s.u, s.v, s.w, s.E, s.E0, s.life,
+                        s.ctr, s.stream, s.cell, s.group, s.desc, s.stype, s.key};
     return f[k];
   }
 };
@@ -110,10 +112,14 @@ struct bh_handle {
   SourceInst* d_inst2 = nullptr;  // [census, new] source records, E_tot set on the device
   DArr<double> sum_part;          // block partials of the deterministic sums
   // comb (bh_comb_census): sort buffers, keep flags, per-cell energies
-  DArr<uint32_t> comb_keys, comb_idx, comb_sorted;
+  DArr<uint32_t> comb_keys, comb_idx, comb_sorted, comb_perm, comb_rank_cell;
   DArr<uint8_t> comb_keep, comb_tmp;
   DArr<double> comb_cellE;
   DArr<unsigned long long> comb_ctr;
+  DArr<uint64_t> cflag;       // census order: survivor flags by transport index, then ranks
+  DArr<HoleRec> hrec;         // census slots a wave took and left unused
+  DArr<uint64_t> hole_len, hole_off, tail_live, tail_rank;  // hole fill
+  CensusAcc* d_cacc = nullptr;  // census totals of a census-writing launch
   uint64_t serial_etot_steps = 0; // steps whose E_tot needed the serial sum
   uint32_t dims[3] = {0, 0, 0};
   DArr<CellRec> rec;
@@ -133,8 +139,7 @@ struct bh_handle {
   DArr<double> abs_E, track_E;
 
   // scratch
-  DArr<uint64_t> counts, off, tiles, tiles2;
-  DArr<CensusPartial> parts;
+  DArr<uint64_t> counts, off, tiles;
   DArr<AosPhoton> aos;
   DArr<double> adj_g, grad_sa, grad_f;  // adjoint: dL/d abs_E in, gradients out
 
@@ -236,7 +241,7 @@ int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
                 (unsigned long long)cap);
   PhotonSoA ns = PhotonStore::carve(slab, cap);
   if (keep && s.slab) {
-    for (int k = 0; k < 15; ++k)
+    for (int k = 0; k < kFields; ++k)
       HIPCHK(h, hipMemcpyAsync(PhotonStore::field(ns, k), PhotonStore::field(s.soa, k),
                                keep * kFieldBytes[k], hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -430,18 +435,19 @@ bool use_lds_tally(const bh_handle* h, bool persistent) {
 // device, in h->d_ctrs->replays); same tally sink as the kernel before it.
 template <class T>
 int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const ReplayList& rl,
-                  T t, size_t shm) {
+                  T t, size_t shm, const CensusOut& co, const StepParams* P) {
   k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(cells, W, rl, h->ctr_hi,
-                                                                     t, h->d_ctrs);
+                                                                     t, h->d_ctrs, co, P);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
 }
 
 // Transport of W[0, n) with the configured kernel; counters in h->d_ctrs.
 // With P (a step being captured as a graph) the persistent kernel reads n on
-// the device and n only sizes the grid and buffers.
-int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n,
-                     const StepParams* P = nullptr) {
+// the device and n only sizes the grid and buffers.  co: census output
+// (co.on = 0: photons stay in place in W).
+int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepParams* P,
+                     const CensusOut& co) {
   if (n == 0) return BH_OK;
   const DevCells cells = dev_cells(h);
   const int block = h->opt.block_size;
@@ -465,19 +471,19 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n,
       if (fp32) {
         if (lat)
           k_transport_history<T, float, true><<<grid_for(n, block), block, shm, h->stream>>>(
-              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
         else
           k_transport_history<T, float, false><<<grid_for(n, block), block, shm, h->stream>>>(
-              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
         HIPCHK(h, hipGetLastError());
-        return launch_replay(h, cells, W, rl, t, shm);
+        return launch_replay(h, cells, W, rl, t, shm, co, P);
       }
       if (lat)
         k_transport_history<T, double, true><<<grid_for(n, block), block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
       else
         k_transport_history<T, double, false><<<grid_for(n, block), block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl);
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
       HIPCHK(h, hipGetLastError());
       return BH_OK;
     });
@@ -494,19 +500,19 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n,
     if (fp32) {
       if (lat)
         k_transport_persistent<T, float, true><<<grid, block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
       else
         k_transport_persistent<T, float, false><<<grid, block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
       HIPCHK(h, hipGetLastError());
-      return launch_replay(h, cells, W, rl, t, shm);
+      return launch_replay(h, cells, W, rl, t, shm, co, P);
     }
     if (lat)
       k_transport_persistent<T, double, true><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
     else
       k_transport_persistent<T, double, false><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   });
@@ -517,6 +523,90 @@ int zero_tallies_and_counters(bh_handle* h) {
   k_fill_f64<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(h->track_E.p, h->n_cells, 0.0);
   k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_ctrs),
                                       sizeof(DevCounters) / 8);
+  HIPCHK(h, hipGetLastError());
+  return BH_OK;
+}
+
+// Capacities of a census-writing launch over at most n photons.  Every
+// wave that writes census photons leaves at most kCenChunk - 1 unused slots
+// (its last chunk), and holds one hole record.  A persistent or replay grid
+// has a fixed number of waves whatever n; a history launch has one wave per
+// 64 photons, each needing at most one chunk.
+struct CensusSizes {
+  uint64_t slots, rec_cap, hole_cap;
+};
+CensusSizes census_sizes(const bh_handle* h, uint64_t n) {
+  const uint64_t block = (uint64_t)h->opt.block_size;
+  const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
+  const uint64_t w_rep = fp32 ? (uint64_t)h->n_cu * 4 * 256 / kWave : 0;
+  CensusSizes z;
+  if (h->opt.algorithm == BH_ALGO_PERSISTENT) {
+    const uint64_t bpc = h->opt.blocks_per_cu > 0 ? (uint64_t)h->opt.blocks_per_cu : 8;
+    z.rec_cap = (uint64_t)h->n_cu * bpc * block / kWave + w_rep;
+    z.slots = n + kCenChunk * z.rec_cap;
+  } else {
+    const uint64_t w_main = div_up(std::max<uint64_t>(n, 1), block) * block / kWave;
+    z.rec_cap = w_main + w_rep;
+    z.slots = kCenChunk * w_main + (fp32 ? n + kCenChunk * w_rep : 0);
+  }
+  z.hole_cap = kCenChunk * z.rec_cap;
+  return z;
+}
+
+// Buffers and the output record of a census-writing launch over the n
+// photons of W = [census | new]: C has room for all of them and the holes
+// the waves leave, the flags by transport index are cleared, the totals
+// zeroed.  Fill kernels, no memset, so the same code records into a graph
+// (P: sizes read on the device; cap bounds n).
+int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co) {
+  const CensusSizes z = census_sizes(h, cap);
+  TRY(grow_store(h, h->C, z.slots, 0));
+  TRY(ensure(h, h->cflag, std::max<uint64_t>(cap, 1)));
+  TRY(ensure(h, h->hrec, z.rec_cap));
+  k_fill_u64<<<grid_for(std::max<uint64_t>(cap, 1), 256), 256, 0, h->stream>>>(
+      reinterpret_cast<unsigned long long*>(h->cflag.p), std::max<uint64_t>(cap, 1));
+  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_cacc),
+                                      sizeof(CensusAcc) / 8);
+  HIPCHK(h, hipGetLastError());
+  co = CensusOut{h->C.soa, h->cflag.p, h->d_cacc, h->hrec.p, kC * next_dt,
+                 h->n_cen,  h->n_new,   h->C.cap,  z.rec_cap,  1};
+  return BH_OK;
+}
+
+// After the launch: the holes below n_census filled from the photons above
+// it, each census photon's rank in the reference's order (exclusive prefix
+// of the flags over transport index), the census energy (deterministic sum
+// over C) and the totals, into h->d_ctot.
+int census_finish(bh_handle* h, uint64_t cap, const uint64_t* n_ptr) {
+  const CensusSizes z = census_sizes(h, cap);
+  TRY(ensure(h, h->hole_len, z.rec_cap));
+  TRY(ensure(h, h->hole_off, z.rec_cap));
+  TRY(ensure(h, h->tail_live, z.hole_cap));
+  TRY(ensure(h, h->tail_rank, z.hole_cap));
+  CensusAcc* acc = h->d_cacc;
+  const unsigned g_rec = std::min<unsigned>(grid_for(z.rec_cap, 256), 1024);
+  const unsigned g_hole = std::min<unsigned>(grid_for(z.hole_cap, 256), 2048);
+  k_hole_heads<<<g_rec, 256, 0, h->stream>>>(h->hrec.p, acc, z.rec_cap, h->hole_len.p);
+  TRY(scan_u64(h, h->hole_len.p, z.rec_cap, h->hole_off.p, 3,
+               reinterpret_cast<const uint64_t*>(&acc->n_rec)));
+  k_hole_tail<<<g_hole, 256, 0, h->stream>>>(h->C.soa.key, acc, z.hole_cap, h->tail_live.p);
+  TRY(scan_u64(h, h->tail_live.p, z.hole_cap, h->tail_rank.p, 3,
+               reinterpret_cast<const uint64_t*>(&acc->n_holes)));
+  k_hole_fill<<<g_hole, 256, 0, h->stream>>>(h->C.soa, acc, z.hole_cap, h->tail_live.p,
+                                             h->tail_rank.p, h->hrec.p, h->hole_off.p, z.rec_cap);
+  HIPCHK(h, hipGetLastError());
+  const uint64_t n = std::max<uint64_t>(cap, 1);
+  TRY(ensure(h, h->off, n));
+  TRY(scan_u64(h, h->cflag.p, n, h->off.p, 2, n_ptr));
+  const unsigned long long* cnt = &acc->n_census;
+  k_census_rekey<<<std::min<unsigned>(grid_for(n, 256), 8192), 256, 0, h->stream>>>(
+      h->C.soa.key, h->off.p, cnt, n);
+  const unsigned nb = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(kSumMaxBlocks,
+                                                                         div_up(n, 4096)));
+  TRY(ensure(h, h->sum_part, (size_t)8 * nb));
+  k_census_E_partials<<<nb, 256, 0, h->stream>>>(h->C.soa.E, cnt, n, h->sum_part.p);
+  k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_ctot->census_E);
+  k_census_pack<<<1, 1, 0, h->stream>>>(acc, h->d_ctot);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
 }
@@ -581,6 +671,7 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
   for (auto& e : h->ev) HIPCHK(h, hipEventCreate(&e));
   HIPCHK(h, hipMalloc(&h->d_total, 4 * sizeof(uint64_t)));
   HIPCHK(h, hipMalloc(&h->d_ctot, sizeof(CensusPartial)));
+  HIPCHK(h, hipMalloc(&h->d_cacc, sizeof(CensusAcc)));
   HIPCHK(h, hipMalloc(&h->d_ctrs, sizeof(DevCounters)));
   HIPCHK(h, hipMalloc(&h->d_queue, sizeof(PersistentQueue)));
   HIPCHK(h, hipHostMalloc(&h->h_pinned, 4096, hipHostMallocDefault));
@@ -609,8 +700,6 @@ void bh_destroy(bh_handle* h) {
   release(h->counts);
   release(h->off);
   release(h->tiles);
-  release(h->tiles2);
-  release(h->parts);
   release(h->aos);
   release(h->rec);
   release(h->axes);
@@ -630,12 +719,20 @@ void bh_destroy(bh_handle* h) {
   release(h->comb_keys);
   release(h->comb_idx);
   release(h->comb_sorted);
+  release(h->comb_perm);
+  release(h->comb_rank_cell);
   release(h->comb_keep);
   release(h->comb_tmp);
   release(h->comb_cellE);
   release(h->comb_ctr);
+  release(h->cflag);
+  release(h->hrec);
+  release(h->hole_len);
+  release(h->hole_off);
+  release(h->tail_live);
+  release(h->tail_rank);
   void* ptrs[] = {h->W.slab, h->C.slab, h->d_total, h->d_ctot, h->d_ctrs, h->d_queue,
-                  h->d_phys, h->d_inst2};
+                  h->d_phys, h->d_inst2, h->d_cacc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& g : h->graphs) {
@@ -930,25 +1027,15 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   HIPCHK(h, hipSetDevice(h->device));
   const uint64_t n = h->n_cen + h->n_new;
   TRY(zero_tallies_and_counters(h));
+  // the kernels write the census to C themselves (post_process_photons,
+  // src/post_process_functions.h:32-109); the reference's census order is
+  // restored from the transport indices afterwards
+  CensusOut co;
+  TRY(census_prepare(h, n, next_dt, co));
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-  TRY(launch_transport(h, h->W.soa, n));
+  TRY(launch_transport(h, h->W.soa, n, nullptr, co));
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-
-  // census compaction into C, in the reference's photon order
-  const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kCensusTile), 1);
-  TRY(ensure(h, h->tiles2, n_tiles));
-  TRY(ensure(h, h->parts, n_tiles));
-  TRY(grow_store(h, h->C, n, 0));
-  const unsigned cgrid = (unsigned)std::min<uint64_t>(n_tiles, 8192);
-  k_census_count<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, h->n_new, h->n_cen,
-                                                     h->tiles2.p, h->parts.p, nullptr);
-  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, n_tiles, nullptr, h->d_ctot);
-  // exclusive scan of per-tile census counts -> tile offsets
-  TRY(ensure(h, h->off, n_tiles));
-  TRY(scan_u64(h, h->tiles2.p, n_tiles, h->off.p, 2));
-  k_census_scatter<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, h->n_new, h->n_cen, h->off.p,
-                                                       h->C.soa, kC * next_dt, nullptr);
-  HIPCHK(h, hipGetLastError());
+  TRY(census_finish(h, n, nullptr));
   HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
 
   // small readback: census totals + counters
@@ -968,6 +1055,12 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   DevCounters ctrs;
   std::memcpy(&tot, hp, sizeof tot);
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  if (tot.overflow) {  // excluded by the sizing of C and the hole records
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE, "census output exceeded its capacity (%llu slots)",
+                (unsigned long long)h->C.cap);
+  }
 
   std::swap(h->W, h->C);
   h->n_cen = tot.n_census;
@@ -1016,16 +1109,19 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
   return {u(h->W.slab),      u(h->C.slab),    (uintptr_t)h->W.cap,  (uintptr_t)h->C.cap,
           u(h->geo.p),       u(h->sigma.p),   u(h->next.p),         u(h->em.p),
           u(h->src.p),       u(h->face.p),    u(h->counts.p),       u(h->off.p),
-          u(h->tiles.p),     u(h->tiles2.p),  u(h->parts.p),        u(h->abs_E.p),
+          u(h->tiles.p),     u(h->abs_E.p),
           u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
           u(h->h_tally),     (uintptr_t)h->n_cells, (uintptr_t)h->G,
           u(h->rec.p),       u(h->axes.p),    (uintptr_t)h->lattice,
+          u(h->cflag.p),     u(h->d_cacc),    u(h->sum_part.p),
+          u(h->hrec.p),      u(h->hole_len.p), u(h->hole_off.p),   u(h->tail_live.p),
+          u(h->tail_rank.p),
           (uintptr_t)h->gray_sigma_a, (uintptr_t)use_hash_tally(h), (uintptr_t)h->ctr_hi};
 }
 
 // Records source + transport + census + readback on h->stream (being
 // captured).  Sizes come from h->d_params at run time; `cap` bounds them.
-static int record_step(bh_handle* h, uint64_t cap, uint64_t n_tiles_cap) {
+static int record_step(bh_handle* h, uint64_t cap) {
   const uint32_t nc = h->n_cells;
   const DevCells cells = dev_cells(h);
   StepParams* P = h->d_params;
@@ -1040,15 +1136,10 @@ static int record_step(bh_handle* h, uint64_t cap, uint64_t n_tiles_cap) {
       P);
   HIPCHK(h, hipGetLastError());
   TRY(zero_tallies_and_counters(h));
-  TRY(launch_transport(h, h->W.soa, cap, P));
-  const unsigned cgrid = (unsigned)std::min<uint64_t>(n_tiles_cap, 8192);
-  k_census_count<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, 0, 0, 0, h->tiles2.p,
-                                                     h->parts.p, P);
-  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, 0, P, h->d_ctot);
-  TRY(scan_u64(h, h->tiles2.p, n_tiles_cap, h->off.p, 2, &P->n_tiles));
-  k_census_scatter<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, 0, 0, 0, h->off.p, h->C.soa,
-                                                       0.0, P);
-  HIPCHK(h, hipGetLastError());
+  CensusOut co;  // n_cen, n_new and the census life come from P on the device
+  TRY(census_prepare(h, cap, 0.0, co));
+  TRY(launch_transport(h, h->W.soa, cap, P, co));
+  TRY(census_finish(h, cap, &P->n));
   char* hp = static_cast<char*>(h->h_pinned);
   HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
                            h->stream));
@@ -1088,21 +1179,32 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     n_new += host_photons_for(n_user_photons, h->em_host[c], total_E) +
              host_photons_for(n_user_photons, h->src_host[c], total_E);
   const uint64_t n = h->n_cen + n_new;
-  // every buffer the graph touches is sized (and allocated) before capture
-  TRY(grow_store(h, h->W, n, h->n_cen));
-  TRY(grow_store(h, h->C, n, 0));
-  const uint64_t cap = std::max(h->W.cap, h->C.cap);
-  const uint64_t n_tiles_cap = std::max<uint64_t>(div_up(cap, kCensusTile), 1);
+  // every buffer the graph touches is sized (and allocated) before capture;
+  // both stores hold cap photons plus the census slots the waves may leave
+  // unused (a constant of the launch shape), so cap stays put from step to
+  // step while the stores swap
+  const uint64_t slack = census_sizes(h, 0).slots;
+  TRY(grow_store(h, h->W, n + slack, h->n_cen));
+  TRY(grow_store(h, h->C, n + slack, 0));
+  const uint64_t cap = std::min(h->W.cap, h->C.cap) - slack;
+  const CensusSizes z = census_sizes(h, cap);
   TRY(ensure(h, h->counts, nc));
-  TRY(ensure(h, h->off, std::max<uint64_t>(nc, n_tiles_cap)));
-  TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(std::max<uint64_t>(nc, n_tiles_cap),
-                                                     kScanTile), 1)));
-  TRY(ensure(h, h->tiles2, n_tiles_cap));
-  TRY(ensure(h, h->parts, n_tiles_cap));
-  if (h->opt.precision == BH_PRECISION_FP32_TRACK) {
-    TRY(ensure(h, h->rep_idx, cap));
-    TRY(ensure(h, h->rep_first, cap));
-    TRY(ensure(h, h->rep_E, cap));
+  TRY(ensure(h, h->off, std::max<uint64_t>(nc, cap)));
+  TRY(ensure(h, h->cflag, std::max<uint64_t>(cap, 1)));
+  TRY(ensure(h, h->hrec, z.rec_cap));
+  TRY(ensure(h, h->hole_len, z.rec_cap));
+  TRY(ensure(h, h->hole_off, z.rec_cap));
+  TRY(ensure(h, h->tail_live, z.hole_cap));
+  TRY(ensure(h, h->tail_rank, z.hole_cap));
+  TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
+  const uint64_t scan_max = std::max<uint64_t>(std::max<uint64_t>(nc, cap),
+                                               std::max(z.hole_cap, z.rec_cap));
+  TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(scan_max, kScanTile), 1)));
+  if (h->opt.precision == BH_PRECISION_FP32_TRACK) {  // as launch_transport sizes them
+    const uint64_t rep = std::max(h->W.cap, h->C.cap);
+    TRY(ensure(h, h->rep_idx, rep));
+    TRY(ensure(h, h->rep_first, rep));
+    TRY(ensure(h, h->rep_E, rep));
   }
   if (!h->d_params) HIPCHK(h, hipMalloc(&h->d_params, sizeof(StepParams)));
   if (!h->h_params) HIPCHK(h, hipHostMalloc(&h->h_params, sizeof(StepParams), hipHostMallocDefault));
@@ -1114,8 +1216,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     h->h_tally_cap = 2 * (size_t)nc;
   }
   const uint64_t rank_off = n_user_photons * (uint64_t)rank;
-  *h->h_params = StepParams{h->n_cen, n_new, n,
-                            std::max<uint64_t>(div_up(n, kCensusTile), 1), kC * next_dt, dt,
+  *h->h_params = StepParams{h->n_cen, n_new, n, kC * next_dt, dt,
                             SourceInst{n_user_photons, total_E,
                                        10000000000000ull * (uint64_t)cycle + rank_off, ctr_hi,
                                        0, 0}};
@@ -1129,7 +1230,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     g.graph = nullptr;
     g.key.clear();
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = record_step(h, cap, n_tiles_cap);
+    const int rc = record_step(h, cap);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(h->stream, &graph);
     if (rc != BH_OK) {
@@ -1153,6 +1254,12 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
   std::memcpy(&tot, hp, sizeof tot);
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
   std::memcpy(&dev_new, hp + 512, sizeof dev_new);
+  if (tot.overflow) {  // excluded by the sizing of C and the hole records
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE, "census output exceeded its capacity (%llu slots)",
+                (unsigned long long)h->C.cap);
+  }
   if (dev_new != n_new) {  // the host and device counts are the same IEEE arithmetic
     h->n_cen = 0;
     h->n_new = 0;
@@ -1219,7 +1326,7 @@ int bh_census_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
   TRY(ensure(h, h->aos, h->n_cen));
   k_soa_to_aos<<<grid_for(h->n_cen, 256), 256, 0, h->stream>>>(h->W.soa, 0, h->n_cen,
                                                                 h->ctr_hi, dev_cells(h),
-                                                                h->aos.p);
+                                                                h->aos.p, 1, 0);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, h->aos.p, h->n_cen * sizeof(AosPhoton), hipMemcpyDeviceToHost,
                            h->stream));
@@ -1241,11 +1348,11 @@ int bh_photons_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
   if (h->n_new)
     k_soa_to_aos<<<grid_for(h->n_new, 256), 256, 0, h->stream>>>(h->W.soa, h->n_cen, h->n_new,
                                                                   h->ctr_hi, dev_cells(h),
-                                                                  h->aos.p);
+                                                                  h->aos.p, 0, 0);
   if (h->n_cen)
     k_soa_to_aos<<<grid_for(h->n_cen, 256), 256, 0, h->stream>>>(h->W.soa, 0, h->n_cen,
                                                                   h->ctr_hi, dev_cells(h),
-                                                                  h->aos.p + h->n_new);
+                                                                  h->aos.p, 1, h->n_new);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, h->aos.p, total * sizeof(AosPhoton), hipMemcpyDeviceToHost,
                            h->stream));
@@ -1320,12 +1427,13 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     h->src_neff_frac = coll > 0.0 ? (double)S * (double)(S - 1) / coll / (double)h->n_cells : 1.0;
   }
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-  const int rc_t = launch_transport(h, h->C.soa, n);
+  CensusOut in_place{};  // the caller's photons come back in place, in order
+  const int rc_t = launch_transport(h, h->C.soa, n, nullptr, in_place);
   h->src_neff_frac = saved_neff;
   TRY(rc_t);
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
   k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(h->C.soa, 0, n, h->ctr_hi, dev_cells(h),
-                                                        h->aos.p);
+                                                        h->aos.p, 0, 0);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(photons, h->aos.p, n * sizeof(AosPhoton), hipMemcpyDeviceToHost,
                            h->stream));
@@ -1458,6 +1566,7 @@ int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint
                            hipMemcpyDeviceToHost, h->stream));
   // stable sort of the census by cell: each cell's photons in census order
   TRY(ensure(h, h->comb_keys, n));
+  TRY(ensure(h, h->comb_rank_cell, n));
   TRY(ensure(h, h->comb_idx, n));
   TRY(ensure(h, h->comb_sorted, n));
   TRY(ensure(h, h->comb_keep, n));
@@ -1465,15 +1574,21 @@ int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint
   TRY(ensure(h, h->comb_ctr, 2));
   TRY(ensure(h, h->counts, nc));
   TRY(ensure(h, h->off, nc));
+  // the census in the reference's order: perm[rank] = slot of W
+  TRY(ensure(h, h->comb_perm, n));
+  k_iota_u32<<<grid_for(n, 256), 256, 0, h->stream>>>(h->comb_perm.p, n);  // in bounds always
+  k_census_perm<<<grid_for(n, 256), 256, 0, h->stream>>>(h->W.soa.key, n, h->comb_perm.p);
   k_iota_u32<<<grid_for(n, 256), 256, 0, h->stream>>>(h->comb_idx.p, n);
+  k_cells_by_rank<<<grid_for(n, 256), 256, 0, h->stream>>>(h->W.soa.cell, h->comb_perm.p, n,
+                                                           h->comb_rank_cell.p);
   int bits = 1;
   while (bits < 32 && (1ull << bits) < nc) ++bits;
   size_t tmp_bytes = 0;
-  HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, h->W.soa.cell, h->comb_keys.p,
-                                               h->comb_idx.p, h->comb_sorted.p, (int)n, 0, bits,
-                                               h->stream));
+  HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, h->comb_rank_cell.p,
+                                               h->comb_keys.p, h->comb_idx.p, h->comb_sorted.p,
+                                               (int)n, 0, bits, h->stream));
   TRY(ensure(h, h->comb_tmp, tmp_bytes));
-  HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->comb_tmp.p, tmp_bytes, h->W.soa.cell,
+  HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->comb_tmp.p, tmp_bytes, h->comb_rank_cell.p,
                                                h->comb_keys.p, h->comb_idx.p, h->comb_sorted.p,
                                                (int)n, 0, bits, h->stream));
   HIPCHK(h, hipMemsetAsync(h->counts.p, 0, nc * sizeof(uint64_t), h->stream));
@@ -1491,7 +1606,7 @@ int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint
     // comb_ctr: [0] ambiguous flag (low 32 bits), [1] survivors
     HIPCHK(h, hipMemsetAsync(h->comb_ctr.p, 0, 2 * sizeof(unsigned long long), h->stream));
     k_comb_cells<<<grid_for(nc, 256), 256, 0, h->stream>>>(
-        h->comb_sorted.p, reinterpret_cast<const unsigned long long*>(h->off.p),
+        h->comb_sorted.p, h->comb_perm.p, reinterpret_cast<const unsigned long long*>(h->off.p),
         reinterpret_cast<const unsigned long long*>(h->counts.p), nc, h->W.soa.E, cp,
         h->comb_keep.p, h->comb_cellE.p, reinterpret_cast<uint32_t*>(h->comb_ctr.p),
         h->comb_ctr.p + 1);
@@ -1510,7 +1625,8 @@ int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint
   TRY(comb(&amb, &kept));
   const bool serial = amb != 0;
   if (serial) {  // the reference's serial census energy, then comb again
-    k_serial_energy<<<1, kWave, 0, h->stream>>>(h->W.soa.E, n, &h->d_phys->global_E);
+    k_serial_energy<<<1, kWave, 0, h->stream>>>(h->W.soa.E, h->comb_perm.p, n,
+                                                &h->d_phys->global_E);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemcpyAsync(hp + 3072, &h->d_phys->global_E, sizeof(double),
                              hipMemcpyDeviceToHost, h->stream));
@@ -1521,39 +1637,33 @@ int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint
     E_par = E_ser;
     TRY(comb(&amb, &kept));
   }
-  k_comb_commit<<<grid_for(n, 256), 256, 0, h->stream>>>(h->W.soa, n, h->comb_keep.p,
-                                                         h->comb_cellE.p);
-  HIPCHK(h, hipGetLastError());
-  // survivors, in census order, into C (the census compaction, each photon
-  // keeping its life)
-  const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kCensusTile), 1);
-  TRY(ensure(h, h->tiles2, n_tiles));
-  TRY(ensure(h, h->parts, n_tiles));
+  // survivors in census order into C: positions by exclusive scan of the
+  // keep flags, new census ranks as keys
+  TRY(ensure(h, h->cflag, n));
+  TRY(ensure(h, h->off, std::max<uint64_t>(n, nc)));
   TRY(grow_store(h, h->C, n, 0));
-  const unsigned cgrid = (unsigned)std::min<uint64_t>(n_tiles, 8192);
-  k_census_count<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, 0, n, h->tiles2.p, h->parts.p,
-                                                     nullptr);
-  k_census_totals<<<1, 256, 0, h->stream>>>(h->parts.p, n_tiles, nullptr, h->d_ctot);
-  TRY(ensure(h, h->off, std::max<uint64_t>(n_tiles, nc)));
-  TRY(scan_u64(h, h->tiles2.p, n_tiles, h->off.p, 2));
-  k_census_scatter<<<cgrid, kCensusNT, 0, h->stream>>>(h->W.soa, n, 0, n, h->off.p, h->C.soa,
-                                                       -1.0, nullptr);
+  k_keep_flags<<<grid_for(n, 256), 256, 0, h->stream>>>(h->comb_keep.p, n, h->cflag.p);
+  TRY(scan_u64(h, h->cflag.p, n, h->off.p, 2));
+  k_comb_compact<<<grid_for(n, 256), 256, 0, h->stream>>>(h->W.soa, h->comb_perm.p, n,
+                                                          h->comb_keep.p, h->off.p,
+                                                          h->comb_cellE.p, h->C.soa);
   HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
-                           h->stream));
+  // census energy of the survivors (deterministic sum over C)
+  k_energy_partials<<<nb, kSumNT, 0, h->stream>>>(h->C.soa.E, (uint32_t)kept, h->sum_part.p);
+  k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->pre_census);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(hp + 3072, &h->d_phys->pre_census, sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  CensusPartial tot;
-  std::memcpy(&tot, hp, sizeof tot);
-  if (tot.n_census != kept)
-    return fail(h, BH_ERR_DEVICE, "comb kept %llu photons, compaction found %llu",
-                (unsigned long long)kept, (unsigned long long)tot.n_census);
+  double census_after;
+  std::memcpy(&census_after, hp + 3072, sizeof census_after);
   std::swap(h->W, h->C);
   h->n_cen = kept;
   if (rep) {
     rep->n_after = kept;
     rep->comb_E = cp.comb_E;
     rep->census_E_before = E_par;
-    rep->census_E_after = tot.census_E;
+    rep->census_E_after = census_after;
     rep->comb_serial = serial ? 1 : 0;
   }
   return BH_OK;

@@ -21,6 +21,52 @@ struct PhotonSoA {
   uint64_t *ctr, *stream;
   uint32_t *cell, *group;
   uint8_t *desc, *stype;
+  // resident census: the photon's rank in the reference's census order (the
+  // census is stored in the order the transport kernels finish histories)
+  uint32_t* key;
+};
+
+// Census output of a transport launch: each history that ends as census is
+// written by the kernel itself to a slot of C, its transport index t marked
+// in flag[t] -- the reference's census order is the order of t -- and exits,
+// kills and other endings are counted on the fly.  With `on` false the
+// photons stay in place in W (bh_transport_photons).  W = [census (keys =
+// census ranks) | new photons], transport index t = n_new + key for a census
+// photon, phys - n_cen for a new one (src/replicated_driver.h:73-75: new
+// photons first).
+//
+// Slots: a wave takes kCenChunk slots of C at a time with one atomic on
+// n_reserved and fills them in order, so every chunk but a wave's last is
+// full.  The unused tail of each last chunk is a run of holes (key kHole)
+// recorded in rec; after the launch the census photons beyond n_census are
+// moved into the holes below it, leaving C[0, n_census) dense.
+constexpr uint32_t kCenChunk = 64;
+constexpr uint32_t kHole = 0xFFFFFFFFu;
+
+struct HoleRec {
+  uint64_t start, len;
+};
+
+struct CensusAcc {  // zeroed before each launch
+  unsigned long long n_reserved;  // slots of C handed out
+  unsigned long long n_census;    // census photons written
+  unsigned long long n_rec;       // hole records
+  unsigned long long n_holes;     // n_reserved - n_census, set after the launch
+  unsigned long long n_exit, n_killed, n_other;
+  unsigned long long overflow;    // a capacity bound was hit (never, by sizing)
+  double exit_E;
+  double pad[7];
+};
+
+struct CensusOut {
+  PhotonSoA C;
+  uint64_t* flag;   // [n] 1 at the transport index of each census photon
+  CensusAcc* acc;
+  HoleRec* rec;     // [rec_cap]: one per wave that took a chunk
+  double life;      // c * next_dt given to census photons
+  uint64_t n_cen, n_new;
+  uint64_t slots, rec_cap;  // capacity of C, of rec
+  int on;
 };
 
 struct DevCells {
@@ -64,7 +110,6 @@ struct StepParams {
   uint64_t n_cen;       // resident census photons, stored first in W
   uint64_t n_new;       // the step's new photons, stored after them
   uint64_t n;           // n_cen + n_new
-  uint64_t n_tiles;     // census compaction tiles of n (at least 1)
   double census_life;   // c * next_dt
   double dt;
   SourceInst fresh;     // one problem's source parameters
@@ -86,13 +131,14 @@ struct ReplayList {
   double* E;
 };
 
-struct CensusPartial {  // per block of the census compaction
+struct CensusPartial {  // a census-writing launch's totals, read back
   double census_E;
   double exit_E;
   unsigned long long n_census;
   unsigned long long n_exit;
   unsigned long long n_killed;
   unsigned long long n_other;
+  unsigned long long overflow;
 };
 
 // ---------------------------------------------------------------------------
@@ -154,8 +200,10 @@ constexpr int kScanTile = kScanNT * kScanItems;
 // Three-pass exclusive scan of n items.  n is the argument, or *n_ptr when
 // n_ptr is set (sizes that only the device knows, e.g. inside a captured
 // graph); the tile loops are grid-stride, so any grid works.
+// A length read on the device never exceeds the bound the buffers were
+// sized for.
 __device__ __forceinline__ uint64_t scan_n(uint64_t n, const uint64_t* n_ptr) {
-  return n_ptr ? *n_ptr : n;
+  return n_ptr ? (*n_ptr < n ? *n_ptr : n) : n;
 }
 
 __global__ __launch_bounds__(kScanNT) void k_scan_reduce(const uint64_t* __restrict__ in,
@@ -337,6 +385,9 @@ __device__ __forceinline__ void source_one(const DevCells& cells,
   }
   const uint32_t group = (uint32_t)floor(rng.next() * (double)cells.G);
   const uint64_t o = out_base + p;
+  // an initial census (mode 1) is in the reference's order by p; new photons
+  // carry p too
+  W.key[o] = (uint32_t)p;
   W.x[o] = x;
   W.y[o] = y;
   W.z[o] = z;
@@ -604,191 +655,6 @@ struct LdsTally {
 };
 
 // ---------------------------------------------------------------------------
-// Census compaction (post_process_photons, src/post_process_functions.h:
-// 32-59).  Logical order is the reference's photon order -- new photons
-// first, then the previous census -- while W stores [census | new]:
-// logical k maps to physical (k < n_new) ? n_cen + k : k - n_new.
-// Census photons are written in logical order to C with life = c*next_dt.
-// ---------------------------------------------------------------------------
-constexpr int kCensusNT = 256;
-constexpr int kCensusItems = 4;
-constexpr int kCensusTile = kCensusNT * kCensusItems;
-
-__device__ __forceinline__ uint64_t logical_to_phys(uint64_t k, uint64_t n_new, uint64_t n_cen) {
-  return (k < n_new) ? n_cen + k : k - n_new;
-}
-
-// The census kernels loop over tiles (grid-stride); with P set the counts
-// come from the device.
-__device__ __forceinline__ void census_count_tile(const PhotonSoA& W, uint64_t n, uint64_t n_new,
-                                                  uint64_t n_cen, uint64_t tile,
-                                                  uint64_t* __restrict__ tile_count,
-                                                  CensusPartial* __restrict__ part);
-
-__global__ __launch_bounds__(kCensusNT) void k_census_count(PhotonSoA W, uint64_t n,
-                                                            uint64_t n_new, uint64_t n_cen,
-                                                            uint64_t* __restrict__ tile_count,
-                                                            CensusPartial* __restrict__ part,
-                                                            const StepParams* __restrict__ P) {
-  if (P) {
-    n = P->n;
-    n_new = P->n_new;
-    n_cen = P->n_cen;
-  }
-  const uint64_t n_tiles = (n + kCensusTile - 1) / kCensusTile;
-  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x)
-    census_count_tile(W, n, n_new, n_cen, t, tile_count, part);
-  if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {  // one empty tile
-    tile_count[0] = 0;
-    part[0] = CensusPartial{0.0, 0.0, 0, 0, 0, 0};
-  }
-}
-
-__device__ __forceinline__ void census_count_tile(const PhotonSoA& W, uint64_t n, uint64_t n_new,
-                                                  uint64_t n_cen, uint64_t tile,
-                                                  uint64_t* __restrict__ tile_count,
-                                                  CensusPartial* __restrict__ part) {
-  __shared__ double ldsd[kCensusNT / kWave];
-  __shared__ unsigned long long ldsu[kCensusNT / kWave];
-  const uint64_t base = tile * kCensusTile;
-  double cE = 0.0, xE = 0.0;
-  unsigned long long nc = 0, nx = 0, nk = 0, no = 0;
-#pragma unroll
-  for (int it = 0; it < kCensusItems; ++it) {
-    const uint64_t k = base + (uint64_t)it * kCensusNT + threadIdx.x;
-    if (k < n) {
-      const uint64_t i = logical_to_phys(k, n_new, n_cen);
-      const uint8_t d = W.desc[i];
-      if (d == kCensus) {
-        ++nc;
-        cE += W.E[i];
-      } else if (d == kExit) {
-        ++nx;
-        xE += W.E[i];
-      } else if (d == kKilled) {
-        ++nk;
-      } else {
-        ++no;
-      }
-    }
-  }
-  cE = block_sum<double, kCensusNT>(cE, ldsd);
-  xE = block_sum<double, kCensusNT>(xE, ldsd);
-  nc = block_sum<unsigned long long, kCensusNT>(nc, ldsu);
-  nx = block_sum<unsigned long long, kCensusNT>(nx, ldsu);
-  nk = block_sum<unsigned long long, kCensusNT>(nk, ldsu);
-  no = block_sum<unsigned long long, kCensusNT>(no, ldsu);
-  if (threadIdx.x == 0) {
-    tile_count[tile] = nc;
-    part[tile] = CensusPartial{cE, xE, nc, nx, nk, no};
-  }
-}
-
-// Single block: deterministic in-order reduction of the per-tile partials.
-__global__ __launch_bounds__(256) void k_census_totals(const CensusPartial* __restrict__ part,
-                                                       uint64_t n_tiles,
-                                                       const StepParams* __restrict__ P,
-                                                       CensusPartial* __restrict__ out) {
-  if (P) n_tiles = P->n_tiles;
-  __shared__ double ldsd[4];
-  __shared__ unsigned long long ldsu[4];
-  double cE = 0.0, xE = 0.0;
-  unsigned long long nc = 0, nx = 0, nk = 0, no = 0;
-  // each thread sums a contiguous run, then runs combine in thread order
-  const uint64_t per = (n_tiles + 255) / 256;
-  const uint64_t b0 = threadIdx.x * per;
-  for (uint64_t b = b0; b < b0 + per && b < n_tiles; ++b) {
-    cE += part[b].census_E;
-    xE += part[b].exit_E;
-    nc += part[b].n_census;
-    nx += part[b].n_exit;
-    nk += part[b].n_killed;
-    no += part[b].n_other;
-  }
-  cE = block_sum<double, 256>(cE, ldsd);
-  xE = block_sum<double, 256>(xE, ldsd);
-  nc = block_sum<unsigned long long, 256>(nc, ldsu);
-  nx = block_sum<unsigned long long, 256>(nx, ldsu);
-  nk = block_sum<unsigned long long, 256>(nk, ldsu);
-  no = block_sum<unsigned long long, 256>(no, ldsu);
-  if (threadIdx.x == 0) *out = CensusPartial{cE, xE, nc, nx, nk, no};
-}
-
-__device__ __forceinline__ void census_scatter_tile(const PhotonSoA& W, uint64_t n,
-                                                    uint64_t n_new, uint64_t n_cen,
-                                                    uint64_t tile,
-                                                    const uint64_t* __restrict__ tile_off,
-                                                    const PhotonSoA& C, double census_life);
-
-__global__ __launch_bounds__(kCensusNT) void k_census_scatter(
-    PhotonSoA W, uint64_t n, uint64_t n_new, uint64_t n_cen,
-    const uint64_t* __restrict__ tile_off, PhotonSoA C, double census_life,
-    const StepParams* __restrict__ P) {
-  if (P) {
-    n = P->n;
-    n_new = P->n_new;
-    n_cen = P->n_cen;
-    census_life = P->census_life;
-  }
-  const uint64_t n_tiles = (n + kCensusTile - 1) / kCensusTile;
-  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x)
-    census_scatter_tile(W, n, n_new, n_cen, t, tile_off, C, census_life);
-}
-
-__device__ __forceinline__ void census_scatter_tile(const PhotonSoA& W, uint64_t n,
-                                                    uint64_t n_new, uint64_t n_cen,
-                                                    uint64_t tile,
-                                                    const uint64_t* __restrict__ tile_off,
-                                                    const PhotonSoA& C, double census_life) {
-  __shared__ uint32_t lds[kCensusNT / kWave];
-  const uint64_t base = tile * kCensusTile;
-  uint64_t carry = tile_off[tile];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-#pragma unroll
-  for (int it = 0; it < kCensusItems; ++it) {
-    const uint64_t k = base + (uint64_t)it * kCensusNT + threadIdx.x;
-    uint64_t i = 0;
-    bool flag = false;
-    if (k < n) {
-      i = logical_to_phys(k, n_new, n_cen);
-      flag = (W.desc[i] == kCensus);
-    }
-    const uint64_t mask = __ballot(flag);
-    if (lane == 0) lds[wave] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    uint32_t woff = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kCensusNT / kWave; ++w) {
-      const uint32_t c = lds[w];
-      if (w < wave) woff += c;
-      tot += c;
-    }
-    __syncthreads();
-    if (flag) {
-      const uint64_t o = carry + woff + lane_rank(mask);
-      C.x[o] = W.x[i];
-      C.y[o] = W.y[i];
-      C.z[o] = W.z[i];
-      C.u[o] = W.u[i];
-      C.v[o] = W.v[i];
-      C.w[o] = W.w[i];
-      C.E[o] = W.E[i];
-      C.E0[o] = W.E0[i];
-      // census_life < 0: survivors of a comb keep their own life
-      C.life[o] = census_life >= 0.0 ? census_life : W.life[i];
-      C.ctr[o] = W.ctr[i];
-      C.stream[o] = W.stream[i];
-      C.cell[o] = W.cell[i];
-      C.group[o] = W.group[i];
-      C.desc[o] = kCensus;
-      C.stype[o] = W.stype[i];
-    }
-    carry += tot;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // AOS <-> SoA for the reference's 120-byte Photon.
 // ---------------------------------------------------------------------------
 struct AosPhoton {
@@ -807,6 +673,7 @@ __global__ void k_aos_to_soa(const AosPhoton* __restrict__ a, uint64_t n, Photon
   if (i >= n) return;
   const AosPhoton p = a[i];
   const uint64_t o = base + i;
+  W.key[o] = (uint32_t)i;  // an imported census is in the reference's order
   W.x[o] = p.pos[0];
   W.y[o] = p.pos[1];
   W.z[o] = p.pos[2];
@@ -824,11 +691,16 @@ __global__ void k_aos_to_soa(const AosPhoton* __restrict__ a, uint64_t n, Photon
   W.stype[o] = (uint8_t)p.source_type;
 }
 
+// by_key: photon s goes to a[key_off + key[s]] (a resident census, back in
+// the reference's order), else to a[i].
 __global__ void k_soa_to_aos(PhotonSoA W, uint64_t base, uint64_t n, uint64_t ctr_hi,
-                             DevCells cells, AosPhoton* __restrict__ a) {
+                             DevCells cells, AosPhoton* __restrict__ a, int by_key,
+                             uint64_t key_off) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t s = base + i;
+  if (by_key && W.key[s] >= n) return;  // keys of a census are a permutation of [0, n)
+  const uint64_t o = by_key ? key_off + W.key[s] : i;
   AosPhoton p;
   p.cell = W.cell[s];
   // per-instance seed; a photon that left through a PROCESSOR face may name
@@ -851,7 +723,129 @@ __global__ void k_soa_to_aos(PhotonSoA W, uint64_t base, uint64_t n, uint64_t ct
   p.rng[1] = ctr_hi;
   p.rng[2] = W.stream[s];
   p.rng[3] = 0;
-  a[i] = p;
+  a[o] = p;
+}
+
+// Census energy of a census-writing launch, deterministic block partials
+// over C.E[0, *count) (block b sums a fixed range of the capacity `cap`).
+__global__ __launch_bounds__(256) void k_census_E_partials(const double* __restrict__ E,
+                                                           const unsigned long long* __restrict__ count,
+                                                           uint64_t cap, double* __restrict__ part) {
+  __shared__ double lds[256 / kWave];
+  const uint64_t n = *count;
+  const uint64_t per = (cap + gridDim.x - 1) / gridDim.x;
+  const uint64_t lo = (uint64_t)blockIdx.x * per;
+  const uint64_t hi = (lo + per < n) ? lo + per : n;
+  double v = 0.0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) v += E[i];
+  v = block_sum<double, 256>(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x * 8] = v;
+}
+
+// Copies photon `from` of W to slot `to` of C with energy E and key k.
+__device__ __forceinline__ void copy_photon(const PhotonSoA& W, uint64_t from, const PhotonSoA& C,
+                                            uint64_t to, double E, uint32_t k) {
+  C.x[to] = W.x[from];
+  C.y[to] = W.y[from];
+  C.z[to] = W.z[from];
+  C.u[to] = W.u[from];
+  C.v[to] = W.v[from];
+  C.w[to] = W.w[from];
+  C.E[to] = E;
+  C.E0[to] = W.E0[from];
+  C.life[to] = W.life[from];
+  C.ctr[to] = W.ctr[from];
+  C.stream[to] = W.stream[from];
+  C.cell[to] = W.cell[from];
+  C.group[to] = W.group[from];
+  C.desc[to] = W.desc[from];
+  C.stype[to] = W.stype[from];
+  C.key[to] = k;
+}
+
+// The launch's census totals in the readback record (census_E already
+// there).
+__global__ void k_census_pack(const CensusAcc* __restrict__ acc, CensusPartial* __restrict__ out) {
+  out->exit_E = acc->exit_E;
+  out->n_census = acc->n_census;
+  out->n_exit = acc->n_exit;
+  out->n_killed = acc->n_killed;
+  out->n_other = acc->n_other;
+  out->overflow = acc->overflow;
+}
+
+// Hole fill, 1: per record, how many of its holes lie below n_census (those
+// get a photon from above), and the hole count.
+__global__ void k_hole_heads(const HoleRec* __restrict__ rec, CensusAcc* __restrict__ acc,
+                             uint64_t rec_cap, uint64_t* __restrict__ head_len) {
+  const uint64_t nr = acc->n_rec < rec_cap ? acc->n_rec : rec_cap;
+  const uint64_t nc = acc->n_census;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) acc->n_holes = acc->n_reserved - nc;
+  for (uint64_t r = g; r < nr; r += stride) {
+    const uint64_t a = rec[r].start, b = a + rec[r].len;
+    head_len[r] = a < nc ? (b < nc ? b : nc) - a : 0;
+  }
+}
+
+// Hole fill, 2: which slots of [n_census, n_reserved) hold a photon.
+__global__ void k_hole_tail(const uint32_t* __restrict__ key, const CensusAcc* __restrict__ acc,
+                            uint64_t bound, uint64_t* __restrict__ live) {
+  const uint64_t nc = acc->n_census;
+  const uint64_t nh = acc->n_holes < bound ? acc->n_holes : bound;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nh; p += stride)
+    live[p] = key[nc + p] != kHole;
+}
+
+// Hole fill, 3: the j-th photon above n_census moves to the j-th hole below
+// it (holes ordered by record, then by slot).  Order in C is free: keys
+// carry the census order.
+__global__ void k_hole_fill(PhotonSoA C, const CensusAcc* __restrict__ acc, uint64_t bound,
+                            const uint64_t* __restrict__ live, const uint64_t* __restrict__ rank,
+                            const HoleRec* __restrict__ rec, const uint64_t* __restrict__ hole_off,
+                            uint64_t rec_cap) {
+  const uint64_t nc = acc->n_census;
+  const uint64_t nh = acc->n_holes < bound ? acc->n_holes : bound;
+  const uint64_t nr = acc->n_rec < rec_cap ? acc->n_rec : rec_cap;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nh; p += stride) {
+    if (!live[p]) continue;
+    const uint64_t j = rank[p];
+    uint64_t lo = 0, hi = nr;  // last record with hole_off <= j
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (hole_off[mid] <= j)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo == 0) continue;
+    const uint64_t r = lo - 1, to = rec[r].start + (j - hole_off[r]), from = nc + p;
+    if (to < nc) copy_photon(C, from, C, to, C.E[from], C.key[from]);
+  }
+}
+
+// perm[key[s]] = s: the resident census in the reference's order.
+__global__ void k_census_perm(const uint32_t* __restrict__ key, uint64_t n,
+                              uint32_t* __restrict__ perm) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && key[s] < n) perm[key[s]] = (uint32_t)s;
+}
+
+// After a census-writing transport: C.key[s] holds the survivor's transport
+// index t; rank = exclusive prefix of flag over t gives its place in the
+// reference's census order.  count = survivors (on the device); rank holds
+// cap entries.
+__global__ void k_census_rekey(uint32_t* __restrict__ key, const uint64_t* __restrict__ rank,
+                               const unsigned long long* __restrict__ count, uint64_t cap) {
+  const uint64_t n = *count;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+    const uint32_t t = key[s];
+    key[s] = t < cap ? (uint32_t)rank[t] : 0xFFFFFFFFu;
+  }
 }
 
 // ---------------------------------------------------------------------------

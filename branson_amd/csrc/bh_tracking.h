@@ -922,21 +922,122 @@ __device__ __forceinline__ void replay_append(bool want, uint64_t idx, uint32_t 
 // replay.  Returns the segments this kernel applied.
 template <typename R, bool kLat>
 __device__ __forceinline__ uint32_t end_history(const DevCells& cells, const PhotonSoA& W,
-                                                uint64_t i, const Live<R>& L, bool& replay) {
+                                                uint64_t i, const Live<R>& L, bool& replay,
+                                                bool in_place) {
   if constexpr (sizeof(R) == 4) {
     if (L.desc == kReplay) {
       replay = true;
       return L.segs - 1;
     }
   }
-  store_live<R, kLat>(cells, W, i, L);
+  if (in_place) store_live<R, kLat>(cells, W, i, L);
   return L.segs;
+}
+
+// What a lane counts of the histories it ends (census photons go to C), and
+// the wave's chunk of census slots (wave-uniform).
+struct CensusLocal {
+  double exit_E = 0.0;
+  unsigned long long n_exit = 0, n_killed = 0, n_other = 0;
+  unsigned long long n_cen = 0;  // census photons the wave wrote
+  uint64_t cbase = 0;            // first slot of the wave's current chunk
+  uint32_t cfill = kCenChunk;    // slots used in it (kCenChunk: no chunk held)
+};
+
+// Every lane at the same program point.  `ended`: the lane ended a history
+// (not handed to the FP64 replay) this iteration, from physical slot i of W.
+// Census photons go to the next slots of the wave's chunk of C (a new chunk,
+// one atomic, when it fills), with the reference's census life, and their
+// transport index is marked for the census order.
+template <typename R, bool kLat>
+__device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
+                                              const PhotonSoA& W, uint64_t i, const Live<R>& L,
+                                              const CensusOut& co, CensusLocal& loc) {
+  const bool cen = ended && L.desc == kCensus;
+  if (ended && !cen) {  // post_process_photons (src/post_process_functions.h:32-109)
+    if (L.desc == kExit) {
+      loc.exit_E += L.E;
+      ++loc.n_exit;
+    } else if (L.desc == kKilled) {
+      ++loc.n_killed;
+    } else {
+      ++loc.n_other;
+    }
+  }
+  const uint64_t m = __ballot(cen);
+  if (m == 0) return;
+  const uint32_t cnt = (uint32_t)__popcll(m);
+  const uint32_t fill = loc.cfill;
+  const bool take = fill + cnt > kCenChunk;  // wave-uniform
+  uint64_t fresh = 0;
+  if (take) {
+    const int leader = __ffsll((long long)m) - 1;
+    if ((int)(threadIdx.x & (kWave - 1)) == leader)
+      fresh = atomicAdd(&co.acc->n_reserved, (unsigned long long)kCenChunk);
+    fresh = shfl_u64(fresh, leader);
+  }
+  if (cen) {
+    const uint32_t q = fill + lane_rank(m);
+    const uint64_t s = q < kCenChunk ? loc.cbase + q : fresh + (q - kCenChunk);
+    const uint64_t t = (i >= co.n_cen) ? i - co.n_cen : co.n_new + W.key[i];
+    if (s < co.slots) {
+      store_live<R, kLat>(cells, co.C, s, L);
+      co.C.life[s] = co.life;
+      co.C.E0[s] = L.E0;
+      co.C.stream[s] = L.rng.stream;
+      co.C.stype[s] = W.stype[i];
+      co.C.key[s] = (uint32_t)t;
+      if (t < co.n_cen + co.n_new) co.flag[t] = 1;  // every key is set by its producer
+    } else {
+      atomicAdd(&co.acc->overflow, 1ull);
+    }
+  }
+  loc.n_cen += cnt;
+  if (take) {
+    loc.cbase = fresh;
+    loc.cfill = fill + cnt - kCenChunk;
+  } else {
+    loc.cfill = fill + cnt;
+  }
+}
+
+static_assert(kCenChunk == kWave, "a chunk's unused tail is marked one slot per lane");
+
+// End of a launch: the wave's exit energy and counts into the launch totals,
+// its last chunk's unused tail marked and recorded as holes.
+__device__ __forceinline__ void census_flush(const CensusOut& co, const CensusLocal& loc) {
+  double e = loc.exit_E;
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kWave);
+  const unsigned long long nx = wave_sum_u64(loc.n_exit), nk = wave_sum_u64(loc.n_killed),
+                           no = wave_sum_u64(loc.n_other);
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const bool partial = loc.cfill < kCenChunk;  // wave-uniform
+  if (partial && lane >= loc.cfill && loc.cbase + lane < co.slots)
+    co.C.key[loc.cbase + lane] = kHole;  // the unused tail of the last chunk
+  if (lane == 0) {
+    if (nx) {
+      atomicAdd(&co.acc->exit_E, e);
+      atomicAdd(&co.acc->n_exit, nx);
+    }
+    if (nk) atomicAdd(&co.acc->n_killed, nk);
+    if (no) atomicAdd(&co.acc->n_other, no);
+    if (loc.n_cen) atomicAdd(&co.acc->n_census, loc.n_cen);
+    if (partial) {
+      const unsigned long long r = atomicAdd(&co.acc->n_rec, 1ull);
+      if (r < co.rec_cap)
+        co.rec[r] = HoleRec{loc.cbase + loc.cfill, (uint64_t)(kCenChunk - loc.cfill)};
+      else
+        atomicAdd(&co.acc->overflow, 1ull);
+    }
+  }
 }
 
 template <class Tally, typename R, bool kLat>
 BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n,
                                          uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
-                                         ReplayList rl) {
+                                         ReplayList rl, CensusOut co) {
+  CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -950,20 +1051,23 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
     Flush fo;
     fo.has = false;
     fo.kill = 0.0;
-    bool replay = false;
+    bool replay = false, ended = false;
     WaveDraws dr;
     wave_draws(active, L, dr);
     if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr)) {
       stuck = (L.desc == kStuck);
-      segs = end_history<R, kLat>(cells, W, i, L, replay);
+      segs = end_history<R, kLat>(cells, W, i, L, replay, !co.on);
       active = false;
+      ended = true;
     }
     if constexpr (sizeof(R) == 4) replay_append(replay, i, L.first_seg, L.E_handoff, rl, ctrs);
+    if (co.on) census_append<R, kLat>(ended && !replay, cells, W, i, L, co, loc);
     flushes += fo.has;
     tally.push(fo);
   }
   tally.finish(cells.n_cells);
   add_counters(ctrs, segs, flushes, stuck);
+  if (co.on) census_flush(co, loc);
 }
 
 // Persistent form: each wave keeps 64 live photons and refills idle lanes
@@ -975,9 +1079,15 @@ template <class Tally, typename R, bool kLat>
 BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_t n,
                                             uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
                                             PersistentQueue* q, int refill, uint32_t chunk,
-                                            ReplayList rl, const StepParams* P) {
+                                            ReplayList rl, const StepParams* P, CensusOut co) {
   const uint64_t kChunk = chunk;
-  if (P) n = P->n;  // a step captured as a graph
+  if (P) {  // a step captured as a graph
+    n = P->n;
+    co.n_cen = P->n_cen;
+    co.n_new = P->n_new;
+    co.life = P->census_life;
+  }
+  CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
   const int lane = threadIdx.x & (kWave - 1);
@@ -1024,20 +1134,23 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
     Flush fo;
     fo.has = false;
     fo.kill = 0.0;
-    bool replay = false;
+    bool replay = false, ended = false;
     WaveDraws dr;
     wave_draws(active, L, dr);
     if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr)) {
       stuck += (L.desc == kStuck);
-      segs += end_history<R, kLat>(cells, W, idx, L, replay);
+      segs += end_history<R, kLat>(cells, W, idx, L, replay, !co.on);
       active = false;
+      ended = true;
     }
     if constexpr (sizeof(R) == 4) replay_append(replay, idx, L.first_seg, L.E_handoff, rl, ctrs);
+    if (co.on) census_append<R, kLat>(ended && !replay, cells, W, idx, L, co, loc);
     flushes += fo.has;
     tally.push(fo);
   }
   tally.finish(cells.n_cells);
   add_counters(ctrs, segs, flushes, stuck);
+  if (co.on) census_flush(co, loc);
 }
 
 // FP64 replay of the histories the FP32 form stopped: each starts again
@@ -1053,7 +1166,14 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
 // launches.  Loop bounds are block-uniform.
 template <class Tally>
 BH_TRACK_KERNEL void k_transport_replay(DevCells cells, PhotonSoA W, ReplayList rl,
-                                        uint64_t ctr_hi, Tally tally, DevCounters* ctrs) {
+                                        uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
+                                        CensusOut co, const StepParams* P) {
+  if (P) {
+    co.n_cen = P->n_cen;
+    co.n_new = P->n_new;
+    co.life = P->census_life;
+  }
+  CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
   const uint64_t n = ctrs->replays;
@@ -1074,6 +1194,7 @@ BH_TRACK_KERNEL void k_transport_replay(DevCells cells, PhotonSoA W, ReplayList 
       Flush fo;
       fo.has = false;
       fo.kill = 0.0;
+      bool ended = false;
       if (active) {
         if (L.segs + 1 == L.first_seg) {
           L.t_abs = 0.0;
@@ -1093,16 +1214,19 @@ BH_TRACK_KERNEL void k_transport_replay(DevCells cells, PhotonSoA W, ReplayList 
           stuck += (L.desc == kStuck);
           segs += L.segs + 1 - L.first_seg;
           L.E *= L.E_handoff;
-          store_live(cells, W, i, L);
+          if (!co.on) store_live(cells, W, i, L);
           active = false;
+          ended = true;
         }
       }
+      if (co.on) census_append<double, false>(ended, cells, W, i, L, co, loc);
       flushes += fo.has;
       tally.push(fo);
     }
   }
   tally.finish(cells.n_cells);
   add_counters(ctrs, segs, flushes, stuck);
+  if (co.on) census_flush(co, loc);
 }
 
 // Path-replay adjoint of the absorbed-energy tally.  For a loss L with
