@@ -116,7 +116,7 @@ struct bh_handle {
   DArr<uint8_t> comb_keep, comb_tmp;
   DArr<double> comb_cellE;
   DArr<unsigned long long> comb_ctr;
-  DArr<uint64_t> cflag;       // census order: survivor flags by transport index, then ranks
+  DArr<uint64_t> cflag;       // census energies by transport index (nonzero: census)
   DArr<HoleRec> hrec;         // census slots a wave took and left unused
   DArr<uint64_t> hole_len, hole_off, tail_live, tail_rank;  // hole fill
   CensusAcc* d_cacc = nullptr;  // census totals of a census-writing launch
@@ -278,7 +278,7 @@ inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)div_up(n
 // n_ptr the count is read on the device and n only bounds it (buffers and
 // grid are sized for n).
 int scan_u64(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out, int slot,
-             const uint64_t* n_ptr = nullptr) {
+             const uint64_t* n_ptr = nullptr, bool nonzero = false) {
   const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kScanTile), 1);
   // the single-block middle pass loops over the tile sums; keep it short
   if (n_tiles > 65536)
@@ -286,9 +286,15 @@ int scan_u64(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out, int sl
                 (unsigned long long)n);
   TRY(ensure(h, h->tiles, n_tiles));
   const unsigned grid = (unsigned)std::min<uint64_t>(n_tiles, 4096);
-  k_scan_reduce<<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p);
+  if (nonzero)
+    k_scan_reduce<true><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p);
+  else
+    k_scan_reduce<false><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p);
   k_scan_top<<<1, kScanNT, 0, h->stream>>>(h->tiles.p, n, n_ptr, h->d_total + slot);
-  k_scan_down<<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p, out);
+  if (nonzero)
+    k_scan_down<true><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p, out);
+  else
+    k_scan_down<false><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p, out);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
 }
@@ -597,14 +603,15 @@ int census_finish(bh_handle* h, uint64_t cap, const uint64_t* n_ptr) {
   HIPCHK(h, hipGetLastError());
   const uint64_t n = std::max<uint64_t>(cap, 1);
   TRY(ensure(h, h->off, n));
-  TRY(scan_u64(h, h->cflag.p, n, h->off.p, 2, n_ptr));
+  TRY(scan_u64(h, h->cflag.p, n, h->off.p, 2, n_ptr, true));
   const unsigned long long* cnt = &acc->n_census;
   k_census_rekey<<<std::min<unsigned>(grid_for(n, 256), 8192), 256, 0, h->stream>>>(
       h->C.soa.key, h->off.p, cnt, n);
   const unsigned nb = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(kSumMaxBlocks,
                                                                          div_up(n, 4096)));
   TRY(ensure(h, h->sum_part, (size_t)8 * nb));
-  k_census_E_partials<<<nb, 256, 0, h->stream>>>(h->C.soa.E, cnt, n, h->sum_part.p);
+  k_census_E_partials<<<nb, 256, 0, h->stream>>>(reinterpret_cast<const double*>(h->cflag.p),
+                                                 n, n_ptr, h->sum_part.p);
   k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_ctot->census_E);
   k_census_pack<<<1, 1, 0, h->stream>>>(acc, h->d_ctot);
   HIPCHK(h, hipGetLastError());

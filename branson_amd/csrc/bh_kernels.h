@@ -27,8 +27,10 @@ struct PhotonSoA {
 };
 
 // Census output of a transport launch: each history that ends as census is
-// written by the kernel itself to a slot of C, its transport index t marked
-// in flag[t] -- the reference's census order is the order of t -- and exits,
+// written by the kernel itself to a slot of C, its energy stored at its
+// transport index t in flag[t] -- the reference's census order is the order
+// of t, so ranks are a scan of the nonzero flags and the census energy a sum
+// over them in that order -- and exits,
 // kills and other endings are counted on the fly.  With `on` false the
 // photons stay in place in W (bh_transport_photons).  W = [census (keys =
 // census ranks) | new photons], transport index t = n_new + key for a census
@@ -60,7 +62,7 @@ struct CensusAcc {  // zeroed before each launch
 
 struct CensusOut {
   PhotonSoA C;
-  uint64_t* flag;   // [n] 1 at the transport index of each census photon
+  uint64_t* flag;   // [n] census energy bits at each census photon's transport index
   CensusAcc* acc;
   HoleRec* rec;     // [rec_cap]: one per wave that took a chunk
   double life;      // c * next_dt given to census photons
@@ -206,6 +208,14 @@ __device__ __forceinline__ uint64_t scan_n(uint64_t n, const uint64_t* n_ptr) {
   return n_ptr ? (*n_ptr < n ? *n_ptr : n) : n;
 }
 
+// kNz: an item counts 1 when its bits are nonzero (a census energy stored
+// by transport index), else its value.
+template <bool kNz>
+__device__ __forceinline__ uint64_t scan_item(uint64_t v) {
+  return kNz ? (uint64_t)(v != 0) : v;
+}
+
+template <bool kNz>
 __global__ __launch_bounds__(kScanNT) void k_scan_reduce(const uint64_t* __restrict__ in,
                                                          uint64_t n, const uint64_t* n_ptr,
                                                          uint64_t* __restrict__ tile_sum) {
@@ -218,7 +228,7 @@ __global__ __launch_bounds__(kScanNT) void k_scan_reduce(const uint64_t* __restr
 #pragma unroll
     for (int it = 0; it < kScanItems; ++it) {
       const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
-      if (i < n) s += in[i];
+      if (i < n) s += scan_item<kNz>(in[i]);
     }
     s = block_sum<uint64_t, kScanNT>(s, lds);
     if (threadIdx.x == 0) tile_sum[t] = s;
@@ -245,6 +255,7 @@ __global__ __launch_bounds__(kScanNT) void k_scan_top(uint64_t* __restrict__ til
   if (threadIdx.x == 0) *total = carry;
 }
 
+template <bool kNz>
 __global__ __launch_bounds__(kScanNT) void k_scan_down(const uint64_t* __restrict__ in,
                                                        uint64_t n, const uint64_t* n_ptr,
                                                        const uint64_t* __restrict__ tile_off,
@@ -258,7 +269,7 @@ __global__ __launch_bounds__(kScanNT) void k_scan_down(const uint64_t* __restric
 #pragma unroll
     for (int it = 0; it < kScanItems; ++it) {
       const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
-      const uint64_t v = (i < n) ? in[i] : 0;
+      const uint64_t v = (i < n) ? scan_item<kNz>(in[i]) : 0;
       uint64_t tot;
       const uint64_t ex = block_excl_scan<uint64_t, kScanNT>(v, lds, tot);
       if (i < n) out[i] = carry + ex;
@@ -726,13 +737,16 @@ __global__ void k_soa_to_aos(PhotonSoA W, uint64_t base, uint64_t n, uint64_t ct
   a[o] = p;
 }
 
-// Census energy of a census-writing launch, deterministic block partials
-// over C.E[0, *count) (block b sums a fixed range of the capacity `cap`).
+// Census energy of a census-writing launch: deterministic block partials of
+// the census energies stored by transport index (zero where a history did
+// not end as census), so the sum runs in the reference's census order
+// whatever order the waves wrote C in.  Block b sums a fixed range of the
+// bound `cap`; n is cap or, with n_ptr, the count read on the device.
 __global__ __launch_bounds__(256) void k_census_E_partials(const double* __restrict__ E,
-                                                           const unsigned long long* __restrict__ count,
-                                                           uint64_t cap, double* __restrict__ part) {
+                                                           uint64_t cap, const uint64_t* n_ptr,
+                                                           double* __restrict__ part) {
   __shared__ double lds[256 / kWave];
-  const uint64_t n = *count;
+  const uint64_t n = scan_n(cap, n_ptr);
   const uint64_t per = (cap + gridDim.x - 1) / gridDim.x;
   const uint64_t lo = (uint64_t)blockIdx.x * per;
   const uint64_t hi = (lo + per < n) ? lo + per : n;

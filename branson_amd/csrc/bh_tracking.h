@@ -987,7 +987,11 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
       co.C.stream[s] = L.rng.stream;
       co.C.stype[s] = W.stype[i];
       co.C.key[s] = (uint32_t)t;
-      if (t < co.n_cen + co.n_new) co.flag[t] = 1;  // every key is set by its producer
+      // the census energy by transport index marks the photon (-0.0 for a
+      // zero weight: nonzero bits, adds nothing); every key is set by its
+      // producer
+      if (t < co.n_cen + co.n_new)
+        co.flag[t] = (uint64_t)__double_as_longlong(L.E != 0.0 ? (double)L.E : -0.0);
     } else {
       atomicAdd(&co.acc->overflow, 1ull);
     }
