@@ -10,12 +10,13 @@ sigma_s = 1/cm, dt = 0.001 sh, 1e7 photons per step, seed 1234
 A step is one replicated time step.  Its host part (step physics, upload of
 cells and per-cell source energies, tally copy-out, temperature update) runs
 untimed; the timed part is the device phase with every input resident in HBM:
-source sampling + photon transport + census compaction (bh_driver_transport),
-bracketed by device synchronisation.  value = histories transported (new +
+source sampling + photon transport (the kernels write the census) + census
+order and totals (bh_driver_transport), bracketed by device synchronisation.  value = histories transported (new +
 census photons) / summed device-phase seconds of the plain FP64 form (FP64
 tracking that replays the reference's RNG streams and arithmetic; persistent
 waves -- the one-work-item-per-history form gives bit-identical events and is
-reported beside it).
+reported beside it, and so are the same steps with each device phase launched
+as one HIP graph).
 
 The JSON line also carries the chosen form (FP32 tracking, FP64 energies and
 tallies) with its per-cell tally error against the FP64 form, segments/s, the
@@ -62,12 +63,17 @@ def _algorithmic_bytes(form, hist, segs, flushes):
 
 
 def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=DECK,
-              n_photons=N_PHOTONS, n_groups=1, fp32_tol=0.0):
+              n_photons=N_PHOTONS, n_groups=1, fp32_tol=0.0, step_graph=False):
     """Warm up on one driver, then time `steps` steps on a fresh one.  Photon
     buffers are reserved up front (device memory is set-up, not step work):
-    the census grows to ~3.7x the photons per step over five steps."""
+    the census grows to ~3.7x the photons per step over five steps.
+    step_graph: each device phase is one HIP graph launch (captured on the
+    first two graph steps, one graph per parity of the photon stores; the
+    captures are inside the timed steps); transport_ms then times the whole
+    graph."""
     reserve = n_photons * (1 + min(steps, 8))
-    kw = dict(precision=precision, algorithm=algorithm, n_groups=n_groups, fp32_tol=fp32_tol)
+    kw = dict(precision=precision, algorithm=algorithm, n_groups=n_groups, fp32_tol=fp32_tol,
+              step_graph=step_graph)
     if warmup:
         with capi.Driver(deck, t_stop=DT * warmup, **kw) as w:
             w.run()
@@ -345,6 +351,10 @@ def main(argv=None):
                       keep_step1=True)
     value = plain["hist"] / plain["dev_s"]
     bytes_launch, kms, achieved = _roofline("fp64", plain)
+    # the same steps with each device phase as one HIP graph (the two
+    # captures fall inside the timed steps)
+    graph = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT,
+                      step_graph=True)
 
     history = chosen = None
     if not a.no_chosen:
@@ -392,6 +402,11 @@ def main(argv=None):
                    "tally": plain["tally"],
                    "histories": plain["hist"], "segments": plain["segs"]},
         "segments_per_s": plain["segs"] / plain["dev_s"],
+        # photon counts follow temperatures that carry the tally atomics'
+        # round-off, so the graph run may differ by a photon in a rare cell
+        "graph_steps": {"histories_per_s": graph["hist"] / graph["dev_s"],
+                        "ms_per_step": 1e3 * graph["dev_s"] / a.steps,
+                        "histories_segments": [graph["hist"], graph["segs"]]},
         # whole step: host physics, uploads of cells and source energies over
         # PCIe, device phase, tally copy-out and temperature update
         "whole_step_histories_per_s": plain["hist"] / plain["step_s"],

@@ -4,8 +4,8 @@ Every instance of a problems.Batch keeps its own material state; each step
 runs Branson's step physics for all cells at once on the host
 (Mesh::calculate_photon_energy and Mesh::update_temperature,
 src/mesh.h:265-281 and :327-360, emission-only problems), then ONE batched
-source (bh_source_instances), ONE transport launch over the photons of all
-instances and ONE census compaction on the device.  Instances of very
+source (bh_source_instances) and ONE transport launch over the photons of all
+instances, which writes their census on the device.  Instances of very
 different sizes therefore share the GPU: the persistent transport kernel
 refills lanes from whichever instance still has photons.
 """

@@ -117,7 +117,7 @@ typedef struct bh_step_stats {
   uint64_t n_segments;    /* iterations of the tracking loop              */
   uint64_t n_flushes;     /* tally flushes (cell exits and terminations)  */
   double transport_ms;    /* device time of the tracking kernel(s)        */
-  double census_ms;       /* device time of census compaction             */
+  double census_ms;       /* device time of census order and totals       */
   double source_ms;       /* device time of the last bh_source call       */
   uint64_t n_replayed;    /* FP32 tracking: histories whose next decision
                              the FP32 error bound could not separate,
