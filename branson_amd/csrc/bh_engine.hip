@@ -442,8 +442,9 @@ bool use_lds_tally(const bh_handle* h, bool persistent) {
 template <class T>
 int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const ReplayList& rl,
                   T t, size_t shm, const CensusOut& co, const StepParams* P) {
-  k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(cells, W, rl, h->ctr_hi,
-                                                                     t, h->d_ctrs, co, P);
+  k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(W, co, cells, rl,
+                                                                     h->ctr_hi, t, h->d_ctrs,
+                                                                     P);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
 }
@@ -477,19 +478,19 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
       if (fp32) {
         if (lat)
           k_transport_history<T, float, true><<<grid_for(n, block), block, shm, h->stream>>>(
-              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
+              W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
         else
           k_transport_history<T, float, false><<<grid_for(n, block), block, shm, h->stream>>>(
-              cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
+              W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
         HIPCHK(h, hipGetLastError());
         return launch_replay(h, cells, W, rl, t, shm, co, P);
       }
       if (lat)
         k_transport_history<T, double, true><<<grid_for(n, block), block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
+            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
       else
         k_transport_history<T, double, false><<<grid_for(n, block), block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, rl, co);
+            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
       HIPCHK(h, hipGetLastError());
       return BH_OK;
     });
@@ -506,19 +507,19 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
     if (fp32) {
       if (lat)
         k_transport_persistent<T, float, true><<<grid, block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
+            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
       else
         k_transport_persistent<T, float, false><<<grid, block, shm, h->stream>>>(
-            cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
+            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
       HIPCHK(h, hipGetLastError());
       return launch_replay(h, cells, W, rl, t, shm, co, P);
     }
     if (lat)
       k_transport_persistent<T, double, true><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
+          W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
     else
       k_transport_persistent<T, double, false><<<grid, block, shm, h->stream>>>(
-          cells, W, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P, co);
+          W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   });
@@ -1062,6 +1063,11 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   DevCounters ctrs;
   std::memcpy(&tot, hp, sizeof tot);
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  if (ctrs.bad_args) {  // a transport kernel's view of its own arguments disagreed
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE, "transport kernel argument check failed; nothing transported");
+  }
   if (tot.overflow) {  // excluded by the sizing of C and the hole records
     h->n_cen = 0;
     h->n_new = 0;
@@ -1260,6 +1266,11 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
   const char* hp = static_cast<const char*>(h->h_pinned);
   std::memcpy(&tot, hp, sizeof tot);
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  if (ctrs.bad_args) {  // a transport kernel's view of its own arguments disagreed
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE, "transport kernel argument check failed; nothing transported");
+  }
   std::memcpy(&dev_new, hp + 512, sizeof dev_new);
   if (tot.overflow) {  // excluded by the sizing of C and the hole records
     h->n_cen = 0;
@@ -1456,6 +1467,11 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
   h->ctr_hi = h->n_cen ? saved_hi : h->ctr_hi;
   DevCounters ctrs;
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  if (ctrs.bad_args) {  // a transport kernel's view of its own arguments disagreed
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE, "transport kernel argument check failed; nothing transported");
+  }
   for (uint32_t c = 0; c < h->n_cells; ++c) {
     if (abs_E) abs_E[c] += a[c];
     if (track_E) track_E[c] += t[c];
@@ -1531,6 +1547,11 @@ int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
   h->ctr_hi = h->n_cen ? saved_hi : h->ctr_hi;
   DevCounters ctrs;
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  if (ctrs.bad_args) {  // a transport kernel's view of its own arguments disagreed
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE, "transport kernel argument check failed; nothing transported");
+  }
   for (uint32_t c = 0; c < nc; ++c) {
     d_sigma_a[c] += a[c];
     d_fleck[c] += f[c];

@@ -122,6 +122,7 @@ struct DevCounters {
   unsigned long long flushes;
   unsigned long long stuck;
   unsigned long long replays;  // histories handed from FP32 tracking to FP64 replay
+  unsigned long long bad_args;  // launches whose kernel-argument view was wrong (never)
 };
 
 // Histories the FP32-tracking kernels hand over for FP64 replay: the photon

@@ -432,9 +432,13 @@ struct SegInfo {
 };
 
 // ---- FP64: the reference's arithmetic, operation for operation ----------
+// pre: the cell's view loaded before the wave's draws, so that the load's
+// latency hides behind them (the draws' LDS fences keep the compiler from
+// moving it there itself); null: loaded here.
 template <bool kTrack, bool kLat = false, bool kInfo = false>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, Flush& fo,
-                                        SegInfo* si = nullptr, WaveDraws* dr = nullptr) {
+                                        SegInfo* si = nullptr, WaveDraws* dr = nullptr,
+                                        const CellView<kLat>* pre = nullptr) {
   if constexpr (kInfo) {
     si->reemit = 0;
     si->flight = false;
@@ -450,7 +454,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
     return true;
   }
   const uint32_t G = cells.G;
-  const CellView<kLat> cv(cells, L.cell, L.ijk);
+  const CellView<kLat> cv = pre ? *pre : CellView<kLat>(cells, L.cell, L.ijk);
   const double f = cv.f;
   double2 sg = cv.sigma(L.group);
   double total_sigma_s = (1.0 - f) * sg.x + sg.y;
@@ -638,14 +642,15 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
 // of FP64.
 template <bool kTrack, bool kLat = false>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, Flush& fo,
-                                        SegInfo* = nullptr, WaveDraws* dr = nullptr) {
+                                        SegInfo* = nullptr, WaveDraws* dr = nullptr,
+                                        const CellView<kLat>* pre = nullptr) {
   if (++L.segs > kMaxSegments) {
     flush(fo, L);
     L.desc = kStuck;
     return true;
   }
   const uint32_t G = cells.G;
-  const CellView<kLat> cv(cells, L.cell, L.ijk);
+  const CellView<kLat> cv = pre ? *pre : CellView<kLat>(cells, L.cell, L.ijk);
   const double f = cv.f;
   double2 sg = cv.sigma(L.group);
   double total_sigma_s = (1.0 - f) * sg.x + sg.y;
@@ -934,6 +939,63 @@ __device__ __forceinline__ uint32_t end_history(const DevCells& cells, const Pho
   return L.segs;
 }
 
+// The photon stores' 32 field pointers are needed only at refills, history
+// ends and census writes.  Held in scalar registers for the whole kernel
+// they spill, and every iteration pays the reloads.  The transport kernels
+// therefore take the stores as their first two arguments (W at offset 0 of
+// the kernel-argument segment, the census output after it) and read them
+// there where they are used, through a pointer the optimiser cannot see
+// through: scalar loads at the use, nothing held in between.
+using KernargPtr = const __attribute__((address_space(4))) char*;
+constexpr uint32_t kCensusArgOffset =
+    (uint32_t)((sizeof(PhotonSoA) + alignof(CensusOut) - 1) / alignof(CensusOut) *
+               alignof(CensusOut));
+
+// (The host pass parses these device functions too; it has no kernel
+// argument segment, and never runs them.)
+__device__ __forceinline__ PhotonSoA arg_stores() {
+#if __HIP_DEVICE_COMPILE__
+  KernargPtr k = (KernargPtr)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(k));
+  return *(const __attribute__((address_space(4))) PhotonSoA*)k;
+#else
+  return PhotonSoA{};
+#endif
+}
+__device__ __forceinline__ CensusOut arg_census() {
+#if __HIP_DEVICE_COMPILE__
+  KernargPtr k = (KernargPtr)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(k));
+  return *(const __attribute__((address_space(4))) CensusOut*)(k + kCensusArgOffset);
+#else
+  return CensusOut{};
+#endif
+}
+
+// The view must show the by-value arguments.  Every work-item of a launch
+// reaches the same answer; a launch that fails it does nothing, and the
+// host reports the count.
+__device__ __forceinline__ bool args_ok(const PhotonSoA& W, const CensusOut& co,
+                                        DevCounters* ctrs) {
+  const PhotonSoA w = arg_stores();
+  const CensusOut c = arg_census();
+  const bool ok = w.x == W.x && w.key == W.key && c.C.x == co.C.x && c.C.key == co.C.key &&
+                  c.flag == co.flag && c.acc == co.acc && c.rec == co.rec &&
+                  c.slots == co.slots && c.rec_cap == co.rec_cap && c.on == co.on;
+  if (!ok && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctrs->bad_args, 1ull);
+  return ok;
+}
+
+// A launch's counts: the kernel argument's, or the captured step's.
+struct CensusRun {
+  uint64_t n_cen, n_new;
+  double life;
+};
+__device__ __forceinline__ CensusRun census_run(const CensusOut& co, const StepParams* P) {
+  return P ? CensusRun{P->n_cen, P->n_new, P->census_life}
+           : CensusRun{co.n_cen, co.n_new, co.life};
+}
+
 // What a lane counts of the histories it ends (census photons go to C), and
 // the wave's chunk of census slots (wave-uniform).
 struct CensusLocal {
@@ -950,22 +1012,22 @@ struct CensusLocal {
 // one atomic, when it fills), with the reference's census life, and their
 // transport index is marked for the census order.
 template <typename R, bool kLat>
-__device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
-                                              const PhotonSoA& W, uint64_t i, const Live<R>& L,
-                                              const CensusOut& co, CensusLocal& loc) {
+__device__ __forceinline__ void census_append(bool ended, const DevCells& cells, uint64_t i,
+                                              const Live<R>& L, const CensusRun& cr,
+                                              CensusLocal& loc) {
   const bool cen = ended && L.desc == kCensus;
-  if (ended && !cen) {  // post_process_photons (src/post_process_functions.h:32-109)
-    if (L.desc == kExit) {
-      loc.exit_E += L.E;
-      ++loc.n_exit;
-    } else if (L.desc == kKilled) {
-      ++loc.n_killed;
-    } else {
-      ++loc.n_other;
-    }
-  }
+  // post_process_photons (src/post_process_functions.h:32-109).  Counted
+  // with selects: an if-chain over the three counters becomes a
+  // pointer-selected increment that puts them in scratch memory.
+  const bool ex = ended && L.desc == kExit, kl = ended && L.desc == kKilled;
+  loc.exit_E += ex ? (double)L.E : 0.0;
+  loc.n_exit += ex ? 1u : 0u;
+  loc.n_killed += kl ? 1u : 0u;
+  loc.n_other += (ended && !cen && !ex && !kl) ? 1u : 0u;
   const uint64_t m = __ballot(cen);
   if (m == 0) return;
+  const CensusOut co = arg_census();
+  const PhotonSoA W = arg_stores();
   const uint32_t cnt = (uint32_t)__popcll(m);
   const uint32_t fill = loc.cfill;
   const bool take = fill + cnt > kCenChunk;  // wave-uniform
@@ -979,10 +1041,10 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
   if (cen) {
     const uint32_t q = fill + lane_rank(m);
     const uint64_t s = q < kCenChunk ? loc.cbase + q : fresh + (q - kCenChunk);
-    const uint64_t t = (i >= co.n_cen) ? i - co.n_cen : co.n_new + W.key[i];
+    const uint64_t t = (i >= cr.n_cen) ? i - cr.n_cen : cr.n_new + W.key[i];
     if (s < co.slots) {
       store_live<R, kLat>(cells, co.C, s, L);
-      co.C.life[s] = co.life;
+      co.C.life[s] = cr.life;
       co.C.E0[s] = L.E0;
       co.C.stream[s] = L.rng.stream;
       co.C.stype[s] = W.stype[i];
@@ -990,7 +1052,7 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
       // the census energy by transport index marks the photon (-0.0 for a
       // zero weight: nonzero bits, adds nothing); every key is set by its
       // producer
-      if (t < co.n_cen + co.n_new)
+      if (t < cr.n_cen + cr.n_new)
         co.flag[t] = (uint64_t)__double_as_longlong(L.E != 0.0 ? (double)L.E : -0.0);
     } else {
       atomicAdd(&co.acc->overflow, 1ull);
@@ -1009,7 +1071,8 @@ static_assert(kCenChunk == kWave, "a chunk's unused tail is marked one slot per 
 
 // End of a launch: the wave's exit energy and counts into the launch totals,
 // its last chunk's unused tail marked and recorded as holes.
-__device__ __forceinline__ void census_flush(const CensusOut& co, const CensusLocal& loc) {
+__device__ __forceinline__ void census_flush(const CensusLocal& loc) {
+  const CensusOut co = arg_census();
   double e = loc.exit_E;
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kWave);
@@ -1038,10 +1101,12 @@ __device__ __forceinline__ void census_flush(const CensusOut& co, const CensusLo
 }
 
 template <class Tally, typename R, bool kLat>
-BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n,
-                                         uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
-                                         ReplayList rl, CensusOut co) {
+BH_TRACK_KERNEL void k_transport_history(const PhotonSoA W, const CensusOut co,
+                                         DevCells cells, uint64_t n, uint64_t ctr_hi,
+                                         Tally tally, DevCounters* ctrs, ReplayList rl) {
+  if (!args_ok(W, co, ctrs)) return;
   CensusLocal loc;
+  const CensusRun cr = census_run(co, nullptr);
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1056,22 +1121,24 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
     fo.has = false;
     fo.kill = 0.0;
     bool replay = false, ended = false;
+    // the cell's record is loaded before the draws (cell 0 for an idle lane)
+    const CellView<kLat> cv(cells, active ? L.cell : 0u, L.ijk);
     WaveDraws dr;
     wave_draws(active, L, dr);
-    if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr)) {
+    if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr, &cv)) {
       stuck = (L.desc == kStuck);
-      segs = end_history<R, kLat>(cells, W, i, L, replay, !co.on);
+      segs = end_history<R, kLat>(cells, arg_stores(), i, L, replay, !co.on);
       active = false;
       ended = true;
     }
     if constexpr (sizeof(R) == 4) replay_append(replay, i, L.first_seg, L.E_handoff, rl, ctrs);
-    if (co.on) census_append<R, kLat>(ended && !replay, cells, W, i, L, co, loc);
+    if (co.on) census_append<R, kLat>(ended && !replay, cells, i, L, cr, loc);
     flushes += fo.has;
     tally.push(fo);
   }
   tally.finish(cells.n_cells);
   add_counters(ctrs, segs, flushes, stuck);
-  if (co.on) census_flush(co, loc);
+  if (co.on) census_flush(loc);
 }
 
 // Persistent form: each wave keeps 64 live photons and refills idle lanes
@@ -1080,17 +1147,15 @@ BH_TRACK_KERNEL void k_transport_history(DevCells cells, PhotonSoA W, uint64_t n
 // atomic per chunk histories instead of one per refill.  The loop ends
 // when the queue is drained and no lane holds a photon, so every wave exits.
 template <class Tally, typename R, bool kLat>
-BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_t n,
+BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut co,
+                                            DevCells cells, uint64_t n,
                                             uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
                                             PersistentQueue* q, int refill, uint32_t chunk,
-                                            ReplayList rl, const StepParams* P, CensusOut co) {
+                                            ReplayList rl, const StepParams* P) {
+  if (!args_ok(W, co, ctrs)) return;
   const uint64_t kChunk = chunk;
-  if (P) {  // a step captured as a graph
-    n = P->n;
-    co.n_cen = P->n_cen;
-    co.n_new = P->n_new;
-    co.life = P->census_life;
-  }
+  if (P) n = P->n;  // a step captured as a graph
+  const CensusRun cr = census_run(co, P);
   CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
@@ -1125,7 +1190,7 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
         const bool take = !active && rank < avail;
         if (take) {
           idx = chunk_next + rank;
-          load_live<R, kLat>(cells, W, idx, ctr_hi, L);
+          load_live<R, kLat>(cells, arg_stores(), idx, ctr_hi, L);
           active = true;
         }
         const uint64_t taken = (avail < (uint64_t)n_idle) ? avail : (uint64_t)n_idle;
@@ -1139,22 +1204,24 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
     fo.has = false;
     fo.kill = 0.0;
     bool replay = false, ended = false;
+    // the cell's record is loaded before the draws (cell 0 for an idle lane)
+    const CellView<kLat> cv(cells, active ? L.cell : 0u, L.ijk);
     WaveDraws dr;
     wave_draws(active, L, dr);
-    if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr)) {
+    if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr, &cv)) {
       stuck += (L.desc == kStuck);
-      segs += end_history<R, kLat>(cells, W, idx, L, replay, !co.on);
+      segs += end_history<R, kLat>(cells, arg_stores(), idx, L, replay, !co.on);
       active = false;
       ended = true;
     }
     if constexpr (sizeof(R) == 4) replay_append(replay, idx, L.first_seg, L.E_handoff, rl, ctrs);
-    if (co.on) census_append<R, kLat>(ended && !replay, cells, W, idx, L, co, loc);
+    if (co.on) census_append<R, kLat>(ended && !replay, cells, idx, L, cr, loc);
     flushes += fo.has;
     tally.push(fo);
   }
   tally.finish(cells.n_cells);
   add_counters(ctrs, segs, flushes, stuck);
-  if (co.on) census_flush(co, loc);
+  if (co.on) census_flush(loc);
 }
 
 // FP64 replay of the histories the FP32 form stopped: each starts again
@@ -1169,14 +1236,11 @@ BH_TRACK_KERNEL void k_transport_persistent(DevCells cells, PhotonSoA W, uint64_
 // length is read on the device, so no host round trip separates the two
 // launches.  Loop bounds are block-uniform.
 template <class Tally>
-BH_TRACK_KERNEL void k_transport_replay(DevCells cells, PhotonSoA W, ReplayList rl,
-                                        uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
-                                        CensusOut co, const StepParams* P) {
-  if (P) {
-    co.n_cen = P->n_cen;
-    co.n_new = P->n_new;
-    co.life = P->census_life;
-  }
+BH_TRACK_KERNEL void k_transport_replay(const PhotonSoA W, const CensusOut co,
+                                        DevCells cells, ReplayList rl, uint64_t ctr_hi,
+                                        Tally tally, DevCounters* ctrs, const StepParams* P) {
+  if (!args_ok(W, co, ctrs)) return;
+  const CensusRun cr = census_run(co, P);
   CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
@@ -1190,7 +1254,7 @@ BH_TRACK_KERNEL void k_transport_replay(DevCells cells, PhotonSoA W, ReplayList 
     Live<double> L;
     if (active) {
       i = rl.idx[j];
-      load_live(cells, W, i, ctr_hi, L);
+      load_live(cells, arg_stores(), i, ctr_hi, L);
       L.first_seg = rl.first_seg[j];
       L.E_handoff = rl.E[j];
     }
@@ -1218,19 +1282,19 @@ BH_TRACK_KERNEL void k_transport_replay(DevCells cells, PhotonSoA W, ReplayList 
           stuck += (L.desc == kStuck);
           segs += L.segs + 1 - L.first_seg;
           L.E *= L.E_handoff;
-          if (!co.on) store_live(cells, W, i, L);
+          if (!co.on) store_live(cells, arg_stores(), i, L);
           active = false;
           ended = true;
         }
       }
-      if (co.on) census_append<double, false>(ended, cells, W, i, L, co, loc);
+      if (co.on) census_append<double, false>(ended, cells, i, L, cr, loc);
       flushes += fo.has;
       tally.push(fo);
     }
   }
   tally.finish(cells.n_cells);
   add_counters(ctrs, segs, flushes, stuck);
-  if (co.on) census_flush(co, loc);
+  if (co.on) census_flush(loc);
 }
 
 // Path-replay adjoint of the absorbed-energy tally.  For a loss L with
