@@ -206,14 +206,16 @@ int fail(bh_handle* h, int code, const char* fmt, ...) {
     if (r_ != BH_OK) return r_; \
   } while (0)
 
-// Makes a hold at least n elements (contents are not kept).
+// Makes a hold at least n elements (contents are not kept).  A growing
+// array grows by half again, so one that follows a growing census is
+// reallocated rarely: every hipMalloc / hipFree synchronises the device.
 template <typename T>
 int ensure(bh_handle* h, DArr<T>& a, size_t n) {
   if (n <= a.cap && a.p) return BH_OK;
+  const size_t want = std::max<size_t>({n, a.cap + a.cap / 2, 1});
   if (a.p) HIPCHK(h, hipFree(a.p));
   a.p = nullptr;
   a.cap = 0;
-  const size_t want = std::max<size_t>(n, 1);
   if (hipMalloc(&a.p, want * sizeof(T)) != hipSuccess) {
     a.p = nullptr;
     return fail(h, BH_ERR_ALLOC, "device allocation of %zu bytes failed", want * sizeof(T));
@@ -251,6 +253,13 @@ int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
   s.cap = cap;
   s.soa = ns;
   return BH_OK;
+}
+
+// Arrays indexed by photon are sized from the photon stores' capacity
+// (bh_options.reserve_photons sizes the stores up front), not from the
+// step's count, which grows with the census from step to step.
+uint64_t photon_cap(const bh_handle* h, uint64_t n) {
+  return std::max<uint64_t>({n, h->W.cap, h->C.cap, 1});
 }
 
 DevCells dev_cells(const bh_handle* h) {
@@ -568,7 +577,11 @@ CensusSizes census_sizes(const bh_handle* h, uint64_t n) {
 int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co) {
   const CensusSizes z = census_sizes(h, cap);
   TRY(grow_store(h, h->C, z.slots, 0));
-  TRY(ensure(h, h->cflag, std::max<uint64_t>(cap, 1)));
+  const uint64_t pc = photon_cap(h, cap);
+  TRY(ensure(h, h->cflag, pc));
+  TRY(ensure(h, h->off, pc));
+  TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(pc, kScanTile), 1)));
+  TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
   TRY(ensure(h, h->hrec, z.rec_cap));
   k_fill_u64<<<grid_for(std::max<uint64_t>(cap, 1), 256), 256, 0, h->stream>>>(
       reinterpret_cast<unsigned long long*>(h->cflag.p), std::max<uint64_t>(cap, 1));
@@ -1201,16 +1214,17 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
   TRY(grow_store(h, h->C, n + slack, 0));
   const uint64_t cap = std::min(h->W.cap, h->C.cap) - slack;
   const CensusSizes z = census_sizes(h, cap);
+  const uint64_t pc = photon_cap(h, cap);  // as census_prepare sizes them
   TRY(ensure(h, h->counts, nc));
-  TRY(ensure(h, h->off, std::max<uint64_t>(nc, cap)));
-  TRY(ensure(h, h->cflag, std::max<uint64_t>(cap, 1)));
+  TRY(ensure(h, h->off, std::max<uint64_t>(nc, pc)));
+  TRY(ensure(h, h->cflag, pc));
   TRY(ensure(h, h->hrec, z.rec_cap));
   TRY(ensure(h, h->hole_len, z.rec_cap));
   TRY(ensure(h, h->hole_off, z.rec_cap));
   TRY(ensure(h, h->tail_live, z.hole_cap));
   TRY(ensure(h, h->tail_rank, z.hole_cap));
   TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
-  const uint64_t scan_max = std::max<uint64_t>(std::max<uint64_t>(nc, cap),
+  const uint64_t scan_max = std::max<uint64_t>(std::max<uint64_t>(nc, pc),
                                                std::max(z.hole_cap, z.rec_cap));
   TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(scan_max, kScanTile), 1)));
   if (h->opt.precision == BH_PRECISION_FP32_TRACK) {  // as launch_transport sizes them
