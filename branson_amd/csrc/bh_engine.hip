@@ -140,6 +140,7 @@ struct bh_handle {
 
   // scratch
   DArr<uint64_t> counts, off, tiles;
+  DArr<double> tile_E;        // per-tile census energies of the rank scan
   DArr<AosPhoton> aos;
   DArr<double> adj_g, grad_sa, grad_f;  // adjoint: dL/d abs_E in, gradients out
 
@@ -286,20 +287,30 @@ inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)div_up(n
 // Exclusive scan of in[0, n) into out, total into d_total[slot].  With
 // n_ptr the count is read on the device and n only bounds it (buffers and
 // grid are sized for n).
+// nonzero: an item counts 1 when its bits are nonzero (census energies by
+// transport index); E_total then also receives the items' sum as doubles,
+// summed in a fixed order.
 int scan_u64(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out, int slot,
-             const uint64_t* n_ptr = nullptr, bool nonzero = false) {
+             const uint64_t* n_ptr = nullptr, bool nonzero = false,
+             double* E_total = nullptr) {
   const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kScanTile), 1);
   // the single-block middle pass loops over the tile sums; keep it short
   if (n_tiles > 65536)
     return fail(h, BH_ERR_ARG, "scan of %llu elements exceeds the supported size",
                 (unsigned long long)n);
   TRY(ensure(h, h->tiles, n_tiles));
+  double* tile_E = nullptr;
+  if (nonzero && E_total) {
+    TRY(ensure(h, h->tile_E, n_tiles));
+    tile_E = h->tile_E.p;
+  }
   const unsigned grid = (unsigned)std::min<uint64_t>(n_tiles, 4096);
   if (nonzero)
-    k_scan_reduce<true><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p);
+    k_scan_reduce<true><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p, tile_E);
   else
-    k_scan_reduce<false><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p);
-  k_scan_top<<<1, kScanNT, 0, h->stream>>>(h->tiles.p, n, n_ptr, h->d_total + slot);
+    k_scan_reduce<false><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p, nullptr);
+  k_scan_top<<<1, kScanNT, 0, h->stream>>>(h->tiles.p, n, n_ptr, h->d_total + slot, tile_E,
+                                          tile_E ? E_total : nullptr);
   if (nonzero)
     k_scan_down<true><<<grid, kScanNT, 0, h->stream>>>(in, n, n_ptr, h->tiles.p, out);
   else
@@ -581,6 +592,7 @@ int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co) {
   TRY(ensure(h, h->cflag, pc));
   TRY(ensure(h, h->off, pc));
   TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(pc, kScanTile), 1)));
+  TRY(ensure(h, h->tile_E, std::max<uint64_t>(div_up(pc, kScanTile), 1)));
   TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
   TRY(ensure(h, h->hrec, z.rec_cap));
   k_fill_u64<<<grid_for(std::max<uint64_t>(cap, 1), 256), 256, 0, h->stream>>>(
@@ -617,16 +629,11 @@ int census_finish(bh_handle* h, uint64_t cap, const uint64_t* n_ptr) {
   HIPCHK(h, hipGetLastError());
   const uint64_t n = std::max<uint64_t>(cap, 1);
   TRY(ensure(h, h->off, n));
-  TRY(scan_u64(h, h->cflag.p, n, h->off.p, 2, n_ptr, true));
+  // ranks, and the census energy summed in census order on the way
+  TRY(scan_u64(h, h->cflag.p, n, h->off.p, 2, n_ptr, true, &h->d_ctot->census_E));
   const unsigned long long* cnt = &acc->n_census;
   k_census_rekey<<<std::min<unsigned>(grid_for(n, 256), 8192), 256, 0, h->stream>>>(
       h->C.soa.key, h->off.p, cnt, n);
-  const unsigned nb = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(kSumMaxBlocks,
-                                                                         div_up(n, 4096)));
-  TRY(ensure(h, h->sum_part, (size_t)8 * nb));
-  k_census_E_partials<<<nb, 256, 0, h->stream>>>(reinterpret_cast<const double*>(h->cflag.p),
-                                                 n, n_ptr, h->sum_part.p);
-  k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_ctot->census_E);
   k_census_pack<<<1, 1, 0, h->stream>>>(acc, h->d_ctot);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
@@ -747,6 +754,7 @@ void bh_destroy(bh_handle* h) {
   release(h->comb_cellE);
   release(h->comb_ctr);
   release(h->cflag);
+  release(h->tile_E);
   release(h->hrec);
   release(h->hole_len);
   release(h->hole_off);
@@ -1141,7 +1149,7 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
           u(h->rec.p),       u(h->axes.p),    (uintptr_t)h->lattice,
           u(h->cflag.p),     u(h->d_cacc),    u(h->sum_part.p),
           u(h->hrec.p),      u(h->hole_len.p), u(h->hole_off.p),   u(h->tail_live.p),
-          u(h->tail_rank.p),
+          u(h->tail_rank.p), u(h->tile_E.p),
           (uintptr_t)h->gray_sigma_a, (uintptr_t)use_hash_tally(h), (uintptr_t)h->ctr_hi};
 }
 
@@ -1227,6 +1235,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
   const uint64_t scan_max = std::max<uint64_t>(std::max<uint64_t>(nc, pc),
                                                std::max(z.hole_cap, z.rec_cap));
   TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(scan_max, kScanTile), 1)));
+  TRY(ensure(h, h->tile_E, std::max<uint64_t>(div_up(scan_max, kScanTile), 1)));
   if (h->opt.precision == BH_PRECISION_FP32_TRACK) {  // as launch_transport sizes them
     const uint64_t rep = std::max(h->W.cap, h->C.cap);
     TRY(ensure(h, h->rep_idx, rep));

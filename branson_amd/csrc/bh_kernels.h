@@ -197,7 +197,7 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
 // sums in one block, then scan each tile with its offset.
 // ---------------------------------------------------------------------------
 constexpr int kScanNT = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanNT * kScanItems;
 
 // Three-pass exclusive scan of n items.  n is the argument, or *n_ptr when
@@ -219,32 +219,49 @@ __device__ __forceinline__ uint64_t scan_item(uint64_t v) {
 template <bool kNz>
 __global__ __launch_bounds__(kScanNT) void k_scan_reduce(const uint64_t* __restrict__ in,
                                                          uint64_t n, const uint64_t* n_ptr,
-                                                         uint64_t* __restrict__ tile_sum) {
+                                                         uint64_t* __restrict__ tile_sum,
+                                                         double* __restrict__ tile_E) {
   __shared__ uint64_t lds[kScanNT / kWave];
+  __shared__ double lds_e[kScanNT / kWave];
   n = scan_n(n, n_ptr);
   const uint64_t n_tiles = (n + kScanTile - 1) / kScanTile;
   for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const uint64_t base = t * kScanTile;
     uint64_t s = 0;
+    double e = 0.0;
 #pragma unroll
     for (int it = 0; it < kScanItems; ++it) {
       const uint64_t i = base + (uint64_t)it * kScanNT + threadIdx.x;
-      if (i < n) s += scan_item<kNz>(in[i]);
+      if (i < n) {
+        const uint64_t v = in[i];
+        s += scan_item<kNz>(v);
+        if (kNz) e += __longlong_as_double((long long)v);
+      }
     }
     s = block_sum<uint64_t, kScanNT>(s, lds);
-    if (threadIdx.x == 0) tile_sum[t] = s;
+    if (kNz && tile_E) e = block_sum<double, kScanNT>(e, lds_e);
+    if (threadIdx.x == 0) {
+      tile_sum[t] = s;
+      if (kNz && tile_E) tile_E[t] = e;
+    }
   }
 }
 
 // One block of 256 threads scans the tile sums in place (exclusive) and
-// writes the grand total to *total.
+// writes the grand total to *total; with tile_E, also the sum of the tiles'
+// energies to *E_total (a fixed tree per 256 tiles, chained in tile order:
+// deterministic for a given n).
 __global__ __launch_bounds__(kScanNT) void k_scan_top(uint64_t* __restrict__ tile_sum,
                                                       uint64_t n, const uint64_t* n_ptr,
-                                                      uint64_t* __restrict__ total) {
+                                                      uint64_t* __restrict__ total,
+                                                      const double* __restrict__ tile_E,
+                                                      double* __restrict__ E_total) {
   __shared__ uint64_t lds[kScanNT / kWave];
+  __shared__ double lds_e[kScanNT / kWave];
   n = scan_n(n, n_ptr);
   const uint64_t n_tiles = (n + kScanTile - 1) / kScanTile;
   uint64_t carry = 0;
+  double e_carry = 0.0;
   for (uint64_t b = 0; b < n_tiles; b += kScanNT) {
     const uint64_t i = b + threadIdx.x;
     const uint64_t v = (i < n_tiles) ? tile_sum[i] : 0;
@@ -252,8 +269,12 @@ __global__ __launch_bounds__(kScanNT) void k_scan_top(uint64_t* __restrict__ til
     const uint64_t ex = block_excl_scan<uint64_t, kScanNT>(v, lds, tot);
     if (i < n_tiles) tile_sum[i] = carry + ex;
     carry += tot;
+    if (tile_E) e_carry += block_sum<double, kScanNT>((i < n_tiles) ? tile_E[i] : 0.0, lds_e);
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) {
+    *total = carry;
+    if (tile_E) *E_total = e_carry;
+  }
 }
 
 template <bool kNz>
@@ -736,25 +757,6 @@ __global__ void k_soa_to_aos(PhotonSoA W, uint64_t base, uint64_t n, uint64_t ct
   p.rng[2] = W.stream[s];
   p.rng[3] = 0;
   a[o] = p;
-}
-
-// Census energy of a census-writing launch: deterministic block partials of
-// the census energies stored by transport index (zero where a history did
-// not end as census), so the sum runs in the reference's census order
-// whatever order the waves wrote C in.  Block b sums a fixed range of the
-// bound `cap`; n is cap or, with n_ptr, the count read on the device.
-__global__ __launch_bounds__(256) void k_census_E_partials(const double* __restrict__ E,
-                                                           uint64_t cap, const uint64_t* n_ptr,
-                                                           double* __restrict__ part) {
-  __shared__ double lds[256 / kWave];
-  const uint64_t n = scan_n(cap, n_ptr);
-  const uint64_t per = (cap + gridDim.x - 1) / gridDim.x;
-  const uint64_t lo = (uint64_t)blockIdx.x * per;
-  const uint64_t hi = (lo + per < n) ? lo + per : n;
-  double v = 0.0;
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) v += E[i];
-  v = block_sum<double, 256>(v, lds);
-  if (threadIdx.x == 0) part[blockIdx.x * 8] = v;
 }
 
 // Copies photon `from` of W to slot `to` of C with energy E and key k.
