@@ -348,34 +348,13 @@ __device__ __forceinline__ void source_one(const DevCells& cells,
                                            const double* __restrict__ source_E,
                                            const int8_t* __restrict__ face_of,
                                            const uint64_t* __restrict__ off, uint64_t p,
-                                           const SourceInst& one,
+                                           uint32_t c, const SourceInst& one,
                                            const SourceInst* __restrict__ inst, double dt,
                                            int mode, const PhotonSoA& W, uint64_t out_base);
 
-__global__ __launch_bounds__(256) void k_source_fill(
-    DevCells cells, const double* __restrict__ emission_E, const double* __restrict__ source_E,
-    const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off, uint64_t n_new,
-    SourceInst one, const SourceInst* __restrict__ inst, double dt, int mode, PhotonSoA W,
-    uint64_t out_base, const StepParams* __restrict__ P) {
-  if (P) {
-    out_base = P->n_cen;
-    n_new = P->n_new;
-    dt = P->dt;
-  }
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n_new; p += stride)
-    source_one(cells, emission_E, source_E, face_of, off, p, one, inst, dt, mode, W, out_base);
-}
-
-__device__ __forceinline__ void source_one(const DevCells& cells,
-                                           const double* __restrict__ emission_E,
-                                           const double* __restrict__ source_E,
-                                           const int8_t* __restrict__ face_of,
-                                           const uint64_t* __restrict__ off, uint64_t p,
-                                           const SourceInst& one,
-                                           const SourceInst* __restrict__ inst, double dt,
-                                           int mode, const PhotonSoA& W, uint64_t out_base) {
-  uint32_t lo = 0, hi = cells.n_cells;  // upper_bound(off, p) - 1
+// upper_bound(off[lo, hi), p) - 1: the cell whose photons include p.
+__device__ __forceinline__ uint32_t cell_of_photon(const uint64_t* __restrict__ off, uint64_t p,
+                                                   uint32_t lo, uint32_t hi) {
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
     if (off[mid] <= p)
@@ -383,7 +362,49 @@ __device__ __forceinline__ void source_one(const DevCells& cells,
     else
       hi = mid;
   }
-  const uint32_t c = lo - 1;
+  return lo - 1;
+}
+
+// A block's 256 consecutive photons fall in a few consecutive cells: two
+// waves find the cells of its first and last photon over the whole offset
+// array, and each photon then searches only that range (the same cell the
+// full search gives, in one or two steps instead of ~16 dependent loads).
+__global__ __launch_bounds__(256) void k_source_fill(
+    DevCells cells, const double* __restrict__ emission_E, const double* __restrict__ source_E,
+    const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off, uint64_t n_new,
+    SourceInst one, const SourceInst* __restrict__ inst, double dt, int mode, PhotonSoA W,
+    uint64_t out_base, const StepParams* __restrict__ P) {
+  __shared__ uint32_t s_first, s_last;
+  if (P) {
+    out_base = P->n_cen;
+    n_new = P->n_new;
+    dt = P->dt;
+  }
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p0 = (uint64_t)blockIdx.x * blockDim.x; p0 < n_new; p0 += stride) {
+    const uint64_t p_last = (p0 + blockDim.x < n_new ? p0 + blockDim.x : n_new) - 1;
+    if (threadIdx.x == 0) s_first = cell_of_photon(off, p0, 0u, cells.n_cells);
+    if (threadIdx.x == kWave) s_last = cell_of_photon(off, p_last, 0u, cells.n_cells);
+    __syncthreads();
+    const uint32_t c_first = s_first, c_last = s_last;
+    __syncthreads();
+    const uint64_t p = p0 + threadIdx.x;
+    if (p < n_new) {
+      const uint32_t c = cell_of_photon(off, p, c_first, c_last + 1u);
+      source_one(cells, emission_E, source_E, face_of, off, p, c, one, inst, dt, mode, W,
+                 out_base);
+    }
+  }
+}
+
+__device__ __forceinline__ void source_one(const DevCells& cells,
+                                           const double* __restrict__ emission_E,
+                                           const double* __restrict__ source_E,
+                                           const int8_t* __restrict__ face_of,
+                                           const uint64_t* __restrict__ off, uint64_t p,
+                                           uint32_t c, const SourceInst& one,
+                                           const SourceInst* __restrict__ inst, double dt,
+                                           int mode, const PhotonSoA& W, uint64_t out_base) {
   const uint64_t local = p - off[c];
   const SourceInst sp = inst ? inst[cells.cell_inst ? cells.cell_inst[c] : 0u] : one;
   const uint64_t n_user = sp.n_user;
