@@ -586,6 +586,10 @@ CensusSizes census_sizes(const bh_handle* h, uint64_t n) {
 // zeroed.  Fill kernels, no memset, so the same code records into a graph
 // (P: sizes read on the device; cap bounds n).
 int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co) {
+  // census keys are 32-bit transport indices and ranks, 0xFFFFFFFF marks a hole
+  if (cap >= (uint64_t)kHole)
+    return fail(h, BH_ERR_CAPACITY, "%llu photons in one launch: at most %u are supported",
+                (unsigned long long)cap, kHole - 1u);
   const CensusSizes z = census_sizes(h, cap);
   TRY(grow_store(h, h->C, z.slots, 0));
   const uint64_t pc = photon_cap(h, cap);
