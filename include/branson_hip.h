@@ -219,7 +219,7 @@ int bh_transport(bh_handle *h, double next_dt, double *abs_E, double *track_E,
 
 /* One step's device phase -- bh_source followed by bh_transport, same
  * arguments and results -- as a single HIP graph launch (no counterpart in
- * the reference; the north star's "each time step captured as a graph").
+ * the reference: each time step captured as one graph).
  * Every photon count stays on the device: the host computes the step's
  * photon count exactly (the source's own arithmetic) and the graph's first
  * node copies the step's values in, so no host round trip splits the step.

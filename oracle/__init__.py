@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  This package is the correctness oracle and the timed
 CPU baseline: only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker.
+``cpu_baseline`` leg of ``bench.py`` may import it, and only to check results
+against (or, in the benchmark, to time the CPU).
 The product package (``branson_amd``) never imports it.
 
 The C restatement in ``branson_oracle.c`` follows the reference file:line cited

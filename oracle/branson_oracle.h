@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is the correctness oracle: only the test
  * suite, the smoke check and the CPU-baseline leg of the benchmark may load
- * it, and only as the checker.  The product (branson_amd, libbranson_hip.so)
+ * it, and only to check results against (or to time the CPU).  The product (branson_amd, libbranson_hip.so)
  * never links, loads or calls anything in this directory.
  *
  * Every function below restates one piece of the reference algorithm and
