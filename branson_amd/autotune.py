@@ -21,13 +21,13 @@ from typing import Callable
 from . import capi
 
 CANDIDATES = {
-    "blocks_per_cu": [4, 6, 8, 12, 16],
+    "blocks_per_cu": [0, 4, 8, 12],  # 0: the blocks that stay resident together
     "block_size": [128, 256],
     "refill_threshold": [4, 8, 16, 32],
     "batch_photons": [64, 256, 1024],
     "tally_mode": [0, 1, 4],
 }
-DEFAULTS = {"blocks_per_cu": 8, "block_size": 256, "refill_threshold": 16,
+DEFAULTS = {"blocks_per_cu": 0, "block_size": 256, "refill_threshold": 16,
             "batch_photons": 256, "tally_mode": 0}
 
 

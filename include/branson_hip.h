@@ -62,7 +62,8 @@ typedef struct bh_options {
   int32_t algorithm;     /* BH_ALGO_*                                       */
   int32_t block_size;    /* tracking work-items per block: 64, 128 or 256;
                             0 = default (256)                              */
-  int32_t blocks_per_cu; /* persistent kernel occupancy, 0 = default        */
+  int32_t blocks_per_cu; /* persistent kernel grid per CU, 0 = the blocks  */
+                         /* that stay resident together (at most 8)        */
   int32_t tally_mode;    /* 0 auto (block-private LDS where the mesh fits;
                             else, for group-independent opacities, mode 4
                             when the source energy sits in under a quarter
