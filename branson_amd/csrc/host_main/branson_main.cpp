@@ -2,8 +2,12 @@
 // the MI355X engine and prints the reference's per-step diagnostics
 // (src/main.cc:39-154, src/imc_state.h:264-284).
 //
-//   branson_hip <input.xml> [--fp32] [--persistent] [--groups N]
+//   branson_hip <input.xml> [--fp32] [--history] [--groups N]
 //               [--photons N] [--device D] [--device-physics]
+//
+// Photons are tracked by persistent waves unless --history asks for one
+// work-item per history (the same events, about half the throughput);
+// --persistent is accepted and changes nothing.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -13,15 +17,17 @@
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::printf("Usage: branson_hip <path_to_input_file> [--fp32] [--persistent] "
+    std::printf("Usage: branson_hip <path_to_input_file> [--fp32] [--history] "
                 "[--groups N] [--photons N] [--device D] [--device-physics]\n");
     return EXIT_FAILURE;
   }
   bh_driver_config cfg;
   std::memset(&cfg, 0, sizeof cfg);
+  cfg.engine.algorithm = BH_ALGO_PERSISTENT;
   for (int i = 2; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--fp32")) cfg.engine.precision = BH_PRECISION_FP32_TRACK;
     else if (!std::strcmp(argv[i], "--persistent")) cfg.engine.algorithm = BH_ALGO_PERSISTENT;
+    else if (!std::strcmp(argv[i], "--history")) cfg.engine.algorithm = BH_ALGO_HISTORY;
     else if (!std::strcmp(argv[i], "--groups") && i + 1 < argc) cfg.n_groups = (uint32_t)std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--photons") && i + 1 < argc) cfg.n_photons = std::strtoull(argv[++i], nullptr, 10);
     else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) cfg.device = std::atoi(argv[++i]);
