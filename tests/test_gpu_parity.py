@@ -500,3 +500,38 @@ def test_fp32_tracking_matches_fp64_events_and_tallies(case, algo):
     lhs = a_g.sum() + got["E"][dg == capi.CENSUS].sum() + got["E"][dg == capi.EXIT].sum()
     assert abs(lhs - E0) <= 1e-12 * E0
     assert np.allclose(got["E"], ref["E"], rtol=FP32_TALLY_RTOL, atol=0)
+
+
+@pytest.mark.parametrize("fp32_tol", [1e-6, 1e-2])
+@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT])
+@pytest.mark.parametrize("deck,n_user", [("cube_40.xml", 300_000),
+                                         ("marshak_slab_100.xml", 100_000)])
+def test_fp32_census_equals_fp64_census(deck, n_user, algo, fp32_tol):
+    """The FP32 form writes its census from two launches -- the FP32 pass and
+    the FP64 replay of the histories it handed over, each with its own waves'
+    census slots -- and the census is put back in the reference's order by
+    transport index.  Its events equal the FP64 form's, so the exported
+    census holds the same photons in the same order: cells, groups, RNG
+    counters and descriptors exactly; positions, directions and weights to
+    the FP32 tolerance."""
+    d = _deck(deck)
+    sim = oracle.Sim(d)
+    cells, ph, clk = _step_inputs(sim)
+    census_E = ph["census_E"] if clk["step"] == 1 else None
+    out = {}
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_FP32_TRACK):
+        with _engine(algorithm=algo, precision=prec, fp32_tol=fp32_tol) as e:
+            e.set_mesh_cells(cells)
+            e.source(ph["emission_E"], ph["source_E"], census_E, ph["source_face"], n_user,
+                     ph["total_E"], d.seed, clk["step"], 0, clk["dt"])
+            _, _, st = e.transport(clk["next_dt"])
+            out[prec] = (st, e.census())
+    (s64, c64), (s32, c32) = out[capi.PRECISION_FP64], out[capi.PRECISION_FP32_TRACK]
+    assert s32["n_census"] == s64["n_census"] == len(c64) == len(c32) > 1000
+    for f in ("cell", "group", "rng", "descriptor", "source_type"):
+        assert np.array_equal(c32[f], c64[f]), f
+    assert np.array_equal(c32["life_dx"], c64["life_dx"])  # c * next_dt for every photon
+    tol = max(fp32_tol, 1e-6)
+    assert np.allclose(c32["E"], c64["E"], rtol=4 * tol, atol=0)
+    assert np.allclose(c32["pos"], c64["pos"], rtol=0, atol=1e-5)
+    assert np.allclose(c32["angle"], c64["angle"], rtol=0, atol=1e-5)
