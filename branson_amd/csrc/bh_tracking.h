@@ -876,6 +876,18 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
+// segs per lane; flushes and stuck already summed over the wave.
+__device__ __forceinline__ void add_wave_counters(DevCounters* ctrs, uint32_t segs,
+                                                  unsigned long long flushes,
+                                                  unsigned long long stuck) {
+  const unsigned long long s = wave_sum_u64(segs);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicAdd(&ctrs->segments, s);
+    atomicAdd(&ctrs->flushes, flushes);
+    if (stuck) atomicAdd(&ctrs->stuck, stuck);
+  }
+}
+
 __device__ __forceinline__ void add_counters(DevCounters* ctrs, unsigned long long segs,
                                              unsigned long long flushes,
                                              unsigned long long stuck) {
@@ -998,10 +1010,12 @@ __device__ __forceinline__ CensusRun census_run(const CensusOut& co, const StepP
 
 // What a lane counts of the histories it ends (census photons go to C), and
 // the wave's chunk of census slots (wave-uniform).
+// Everything but the exit energy is wave-uniform (counted from ballots), so
+// it stays in scalar registers.
 struct CensusLocal {
-  double exit_E = 0.0;
-  unsigned long long n_exit = 0, n_killed = 0, n_other = 0;
-  unsigned long long n_cen = 0;  // census photons the wave wrote
+  double exit_E = 0.0;           // per lane
+  uint32_t n_exit = 0, n_killed = 0, n_other = 0;
+  uint32_t n_cen = 0;            // census photons the wave wrote
   uint64_t cbase = 0;            // first slot of the wave's current chunk
   uint32_t cfill = kCenChunk;    // slots used in it (kCenChunk: no chunk held)
 };
@@ -1021,9 +1035,9 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
   // pointer-selected increment that puts them in scratch memory.
   const bool ex = ended && L.desc == kExit, kl = ended && L.desc == kKilled;
   loc.exit_E += ex ? (double)L.E : 0.0;
-  loc.n_exit += ex ? 1u : 0u;
-  loc.n_killed += kl ? 1u : 0u;
-  loc.n_other += (ended && !cen && !ex && !kl) ? 1u : 0u;
+  loc.n_exit += (uint32_t)__popcll(__ballot(ex));
+  loc.n_killed += (uint32_t)__popcll(__ballot(kl));
+  loc.n_other += (uint32_t)__popcll(__ballot(ended && !cen && !ex && !kl));
   const uint64_t m = __ballot(cen);
   if (m == 0) return;
   const CensusOut co = arg_census();
@@ -1037,6 +1051,8 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
     if ((int)(threadIdx.x & (kWave - 1)) == leader)
       fresh = atomicAdd(&co.acc->n_reserved, (unsigned long long)kCenChunk);
     fresh = shfl_u64(fresh, leader);
+    fresh = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(fresh >> 32)) << 32) |
+            __builtin_amdgcn_readfirstlane((uint32_t)fresh);  // wave-uniform
   }
   if (cen) {
     const uint32_t q = fill + lane_rank(m);
@@ -1076,8 +1092,7 @@ __device__ __forceinline__ void census_flush(const CensusLocal& loc) {
   double e = loc.exit_E;
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kWave);
-  const unsigned long long nx = wave_sum_u64(loc.n_exit), nk = wave_sum_u64(loc.n_killed),
-                           no = wave_sum_u64(loc.n_other);
+  const unsigned long long nx = loc.n_exit, nk = loc.n_killed, no = loc.n_other;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const bool partial = loc.cfill < kCenChunk;  // wave-uniform
   if (partial && lane >= loc.cfill && loc.cbase + lane < co.slots)
@@ -1089,7 +1104,7 @@ __device__ __forceinline__ void census_flush(const CensusLocal& loc) {
     }
     if (nk) atomicAdd(&co.acc->n_killed, nk);
     if (no) atomicAdd(&co.acc->n_other, no);
-    if (loc.n_cen) atomicAdd(&co.acc->n_census, loc.n_cen);
+    if (loc.n_cen) atomicAdd(&co.acc->n_census, (unsigned long long)loc.n_cen);
     if (partial) {
       const unsigned long long r = atomicAdd(&co.acc->n_rec, 1ull);
       if (r < co.rec_cap)
@@ -1167,7 +1182,9 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
   uint64_t idx = 0;
   uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
   bool drained = false;
-  unsigned long long flushes = 0, segs = 0, stuck = 0;
+  // segments per lane; flushes and stuck photons counted per wave (scalar)
+  uint32_t segs = 0;
+  unsigned long long flushes = 0, stuck = 0;
   while (true) {
     uint64_t idle = __ballot(!active);
     int n_idle = __popcll(idle);
@@ -1209,18 +1226,18 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
     WaveDraws dr;
     wave_draws(active, L, dr);
     if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, &dr, &cv)) {
-      stuck += (L.desc == kStuck);
-      segs += end_history<R, kLat>(cells, arg_stores(), idx, L, replay, !co.on);
+      segs += (uint32_t)end_history<R, kLat>(cells, arg_stores(), idx, L, replay, !co.on);
       active = false;
       ended = true;
     }
+    stuck += (unsigned long long)__popcll(__ballot(ended && L.desc == kStuck));
     if constexpr (sizeof(R) == 4) replay_append(replay, idx, L.first_seg, L.E_handoff, rl, ctrs);
     if (co.on) census_append<R, kLat>(ended && !replay, cells, idx, L, cr, loc);
-    flushes += fo.has;
+    flushes += (unsigned long long)__popcll(__ballot(fo.has));
     tally.push(fo);
   }
   tally.finish(cells.n_cells);
-  add_counters(ctrs, segs, flushes, stuck);
+  add_wave_counters(ctrs, segs, flushes, stuck);
   if (co.on) census_flush(loc);
 }
 
