@@ -16,6 +16,7 @@ from branson_amd import capi
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _close6(a, b):
@@ -69,3 +70,25 @@ def test_driver_fp32_tracking_runs_hohlraum_30_groups_small():
         scale = r["source_E"] + r["emission_E"] + r["pre_census_E"]
         assert abs(r["rad_conservation"]) < 1e-11 * scale
         assert r["n_segments"] > r["transported"]
+
+
+def test_long_run_keeps_conservation_as_the_census_grows():
+    """20 steps of the gray cube at 1e6 photons per step: the census grows for
+    a few steps and levels off, so the photon stores and every array sized
+    from them grow step after step and are then reused.  Radiation and
+    material conservation hold to 1e-12 at every step, and every step's
+    census comes back as the next step's (transported = census + new)."""
+    deck = os.path.join(ROOT, "branson_amd", "decks", "cube_40_throughput.xml")
+    with capi.Driver(deck, algorithm=capi.ALGO_PERSISTENT, n_photons=1_000_000,
+                     t_stop=0.02) as d:
+        reps = d.run()
+    assert len(reps) == 20
+    prev_census = 0
+    for r in reps:
+        scale = max(r["post_mat_E"], r["absorbed_E"] + r["exit_E"] + r["post_census_E"])
+        assert abs(r["rad_conservation"]) < 1e-12 * scale, r["step"]
+        assert abs(r["mat_conservation"]) < 1e-12 * scale, r["step"]
+        assert r["transported"] > prev_census  # the new photons join the census
+        prev_census = r["census_size"]
+    sizes = [r["census_size"] for r in reps]
+    assert sizes[4] > 2 * sizes[0] and abs(sizes[-1] - sizes[-2]) < 0.05 * sizes[-1]
