@@ -420,33 +420,53 @@ bool use_hash_tally(const bh_handle* h) {
 // every cell's absorption opacity is group-independent (then
 // k_tally_finalize completes the tallies; combined per wave in an LDS hash
 // table where the source is concentrated), two otherwise.
+// Where a transport launch puts its results, and the RNG counter-high word
+// of its photons.  handle_sinks(): the handle's own tallies, counters, queue
+// and replay lists.  additive: the tally arrays already hold values that
+// the launch must add to, so no tally form that rewrites them afterwards
+// (k_tally_finalize) may be chosen.
+struct Sinks {
+  double* abs_E;
+  double* track_E;
+  DevCounters* ctrs;
+  PersistentQueue* queue;
+  ReplayList rl;  // FP32 form; null members: the handle's lists, sized at launch
+  uint64_t ctr_hi;
+  bool additive;
+};
+
+Sinks handle_sinks(const bh_handle* h, uint64_t ctr_hi) {
+  return Sinks{h->abs_E.p, h->track_E.p, h->d_ctrs, h->d_queue,
+               ReplayList{nullptr, nullptr, nullptr}, ctr_hi, false};
+}
+
 template <class Launch>
-int with_tally(bh_handle* h, bool lds, Launch&& launch) {
+int with_tally(bh_handle* h, bool lds, const Sinks& s, Launch&& launch) {
   if (lds) {
     h->last_tally = 2;
-    LdsTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
+    LdsTally t{s.abs_E, s.track_E, nullptr, nullptr, 0, 0};
     return launch(t, (size_t)2 * h->n_cells * sizeof(double));
   }
-  if (h->gray_sigma_a && use_hash_tally(h)) {
+  if (h->gray_sigma_a && !s.additive && use_hash_tally(h)) {
     h->last_tally = 4;
-    HashTally t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0};
+    HashTally t{s.abs_E, s.track_E, nullptr, nullptr, 0};
     TRY(launch(t, (size_t)0));
     k_tally_finalize<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(
-        h->geo.p, h->sigma.p, h->G, h->n_cells, h->abs_E.p, h->track_E.p);
+        h->geo.p, h->sigma.p, h->G, h->n_cells, s.abs_E, s.track_E);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   }
-  if (h->gray_sigma_a) {
+  if (h->gray_sigma_a && !s.additive) {
     h->last_tally = 1;
-    GlobalTally<false> t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
+    GlobalTally<false> t{s.abs_E, s.track_E, nullptr, nullptr, 0, 0};
     TRY(launch(t, (size_t)0));
     k_tally_finalize<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(
-        h->geo.p, h->sigma.p, h->G, h->n_cells, h->abs_E.p, h->track_E.p);
+        h->geo.p, h->sigma.p, h->G, h->n_cells, s.abs_E, s.track_E);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   }
   h->last_tally = 3;
-  GlobalTally<true> t{h->abs_E.p, h->track_E.p, nullptr, nullptr, 0, 0};
+  GlobalTally<true> t{s.abs_E, s.track_E, nullptr, nullptr, 0, 0};
   return launch(t, (size_t)0);
 }
 
@@ -461,10 +481,9 @@ bool use_lds_tally(const bh_handle* h, bool persistent) {
 // device, in h->d_ctrs->replays); same tally sink as the kernel before it.
 template <class T>
 int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const ReplayList& rl,
-                  T t, size_t shm, const CensusOut& co, const StepParams* P) {
+                  T t, size_t shm, const CensusOut& co, const StepParams* P, const Sinks& s) {
   k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(W, co, cells, rl,
-                                                                     h->ctr_hi, t, h->d_ctrs,
-                                                                     P);
+                                                                     s.ctr_hi, t, s.ctrs, P);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
 }
@@ -477,7 +496,7 @@ constexpr int kMaxBlocksPerCu = 8;
 // the device and n only sizes the grid and buffers.  co: census output
 // (co.on = 0: photons stay in place in W).
 int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepParams* P,
-                     const CensusOut& co) {
+                     const CensusOut& co, const Sinks& s) {
   if (n == 0) return BH_OK;
   const DevCells cells = dev_cells(h);
   const int block = h->opt.block_size;
@@ -485,8 +504,10 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
   const bool lds = use_lds_tally(h, persistent);
   const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
   const bool lat = h->lattice;
-  ReplayList rl{nullptr, nullptr, nullptr};
-  if (fp32) {
+  const uint64_t ctr_hi = s.ctr_hi;
+  DevCounters* const ctrs = s.ctrs;
+  ReplayList rl = s.rl;
+  if (fp32 && !rl.idx) {
     // room for every photon of the launch, sized with the photon stores so
     // that a growing census does not reallocate it every step
     const size_t cap = std::max<size_t>(n, std::max(h->W.cap, h->C.cap));
@@ -496,24 +517,24 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
     rl = ReplayList{h->rep_idx.p, h->rep_first.p, h->rep_E.p};
   }
   if (!persistent) {
-    return with_tally(h, lds, [&](auto t, size_t shm) -> int {
+    return with_tally(h, lds, s, [&](auto t, size_t shm) -> int {
       using T = decltype(t);
       if (fp32) {
         if (lat)
           k_transport_history<T, float, true><<<grid_for(n, block), block, shm, h->stream>>>(
-              W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
+              W, co, cells, n, ctr_hi, t, ctrs, rl);
         else
           k_transport_history<T, float, false><<<grid_for(n, block), block, shm, h->stream>>>(
-              W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
+              W, co, cells, n, ctr_hi, t, ctrs, rl);
         HIPCHK(h, hipGetLastError());
-        return launch_replay(h, cells, W, rl, t, shm, co, P);
+        return launch_replay(h, cells, W, rl, t, shm, co, P, s);
       }
       if (lat)
         k_transport_history<T, double, true><<<grid_for(n, block), block, shm, h->stream>>>(
-            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
+            W, co, cells, n, ctr_hi, t, ctrs, rl);
       else
         k_transport_history<T, double, false><<<grid_for(n, block), block, shm, h->stream>>>(
-            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, rl);
+            W, co, cells, n, ctr_hi, t, ctrs, rl);
       HIPCHK(h, hipGetLastError());
       return BH_OK;
     });
@@ -536,31 +557,31 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
     return (unsigned)std::min<uint64_t>((uint64_t)h->n_cu * bpc,
                                         std::max<uint64_t>(div_up(n, block), 1));
   };
-  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_queue),
+  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(s.queue),
                                       sizeof(PersistentQueue) / 8);
-  return with_tally(h, lds, [&](auto t, size_t shm) -> int {
+  return with_tally(h, lds, s, [&](auto t, size_t shm) -> int {
     using T = decltype(t);
     if (fp32) {
       if (lat) {
         auto* k = &k_transport_persistent<T, float, true>;
         k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+            W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
       } else {
         auto* k = &k_transport_persistent<T, float, false>;
         k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-            W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+            W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
       }
       HIPCHK(h, hipGetLastError());
-      return launch_replay(h, cells, W, rl, t, shm, co, P);
+      return launch_replay(h, cells, W, rl, t, shm, co, P, s);
     }
     if (lat) {
       auto* k = &k_transport_persistent<T, double, true>;
       k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-          W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+          W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
     } else {
       auto* k = &k_transport_persistent<T, double, false>;
       k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-          W, co, cells, n, h->ctr_hi, t, h->d_ctrs, h->d_queue, refill, chunk, rl, P);
+          W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
     }
     HIPCHK(h, hipGetLastError());
     return BH_OK;
@@ -1090,7 +1111,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   CensusOut co;
   TRY(census_prepare(h, n, next_dt, co));
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-  TRY(launch_transport(h, h->W.soa, n, nullptr, co));
+  TRY(launch_transport(h, h->W.soa, n, nullptr, co, handle_sinks(h, h->ctr_hi)));
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
   TRY(census_finish(h, n, nullptr));
   HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
@@ -1200,7 +1221,7 @@ static int record_step(bh_handle* h, uint64_t cap) {
   TRY(zero_tallies_and_counters(h));
   CensusOut co;  // n_cen, n_new and the census life come from P on the device
   TRY(census_prepare(h, cap, 0.0, co));
-  TRY(launch_transport(h, h->W.soa, cap, P, co));
+  TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi)));
   TRY(census_finish(h, cap, &P->n));
   char* hp = static_cast<char*>(h->h_pinned);
   HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
@@ -1429,10 +1450,11 @@ int bh_photons_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
   return BH_OK;
 }
 
-static int upload_aos(bh_handle* h, const bh_photon* in, uint64_t n, PhotonStore& dst,
-                      uint64_t keep) {
-  if (n == 0) return BH_OK;
-  const uint64_t hi = in[0].rng[1];
+// Checks the caller's AOS photons against the mesh and returns the RNG
+// counter-high word they share (photon 0's; per instance with instances).
+// A host loop only: no allocation.
+static int check_aos(bh_handle* h, const bh_photon* in, uint64_t n, uint64_t* hi_out) {
+  const uint64_t hi = n ? in[0].rng[1] : 0;
   for (uint64_t i = 0; i < n; ++i) {
     if (h->has_mesh && in[i].cell >= h->n_cells)
       return fail(h, BH_ERR_ARG, "photon %llu: cell %u out of range",
@@ -1450,7 +1472,19 @@ static int upload_aos(bh_handle* h, const bh_photon* in, uint64_t n, PhotonStore
                               "photon 0 (one seed per run is required)",
                   (unsigned long long)i);
   }
-  h->ctr_hi = hi;
+  *hi_out = hi;
+  return BH_OK;
+}
+
+// Uploads AOS photons into store dst (after its first `keep`).  *hi_out
+// receives their counter-high word; the handle's own (that of the resident
+// census) is left as it is.
+static int upload_aos(bh_handle* h, const bh_photon* in, uint64_t n, PhotonStore& dst,
+                      uint64_t keep, uint64_t* hi_out) {
+  *hi_out = h->ctr_hi;
+  if (n == 0) return BH_OK;
+  uint64_t hi = 0;
+  TRY(check_aos(h, in, n, &hi));
   TRY(ensure(h, h->aos, n));
   TRY(grow_store(h, dst, n, keep));
   HIPCHK(h, hipMemcpyAsync(h->aos.p, in, n * sizeof(AosPhoton), hipMemcpyHostToDevice,
@@ -1458,16 +1492,36 @@ static int upload_aos(bh_handle* h, const bh_photon* in, uint64_t n, PhotonStore
   k_aos_to_soa<<<grid_for(n, 256), 256, 0, h->stream>>>(h->aos.p, n, dst.soa, 0);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  *hi_out = hi;
   return BH_OK;
 }
 
 int bh_census_import(bh_handle* h, const bh_photon* in, uint64_t n) {
   if (!h || !h->stream || (n && !in)) return BH_ERR_ARG;
   HIPCHK(h, hipSetDevice(h->device));
-  TRY(upload_aos(h, in, n, h->W, 0));
+  uint64_t hi = 0;
+  TRY(upload_aos(h, in, n, h->W, 0, &hi));
+  h->ctr_hi = hi;
   h->n_cen = n;
   h->n_new = 0;
   return BH_OK;
+}
+
+// Per-photon outcome totals of transported AOS photons (the
+// post_process_photons counts, src/post_process_functions.h:32-59).
+static void outcome_stats(const bh_photon* photons, uint64_t n, bh_step_stats* stats) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint8_t d = photons[i].descriptor[0];
+    if (d == BH_CENSUS) {
+      stats->n_census++;
+      stats->census_E += photons[i].E;
+    } else if (d == BH_EXIT) {
+      stats->n_exit++;
+      stats->exit_E += photons[i].E;
+    } else if (d == BH_KILLED) {
+      stats->n_killed++;
+    }
+  }
 }
 
 int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* abs_E,
@@ -1477,8 +1531,10 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
   HIPCHK(h, hipSetDevice(h->device));
   if (stats) std::memset(stats, 0, sizeof *stats);
   if (n == 0) return BH_OK;
-  const uint64_t saved_hi = h->ctr_hi;
-  TRY(upload_aos(h, photons, n, h->C, 0));
+  // the photons' counter-high word goes to the kernels; the resident
+  // census keeps its own
+  uint64_t hi = 0;
+  TRY(upload_aos(h, photons, n, h->C, 0, &hi));
   TRY(zero_tallies_and_counters(h));
   // tally choice from the photons' cells: S evenly spaced samples, effective
   // number of cells S (S - 1) / sum c (c - 1) (the collision estimator: N for
@@ -1497,11 +1553,11 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
   }
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
   CensusOut in_place{};  // the caller's photons come back in place, in order
-  const int rc_t = launch_transport(h, h->C.soa, n, nullptr, in_place);
+  const int rc_t = launch_transport(h, h->C.soa, n, nullptr, in_place, handle_sinks(h, hi));
   h->src_neff_frac = saved_neff;
   TRY(rc_t);
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-  k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(h->C.soa, 0, n, h->ctr_hi, dev_cells(h),
+  k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(h->C.soa, 0, n, hi, dev_cells(h),
                                                         h->aos.p, 0, 0);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(photons, h->aos.p, n * sizeof(AosPhoton), hipMemcpyDeviceToHost,
@@ -1515,14 +1571,11 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
   HIPCHK(h, hipMemcpyAsync(hp + 256, h->d_ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->ctr_hi = h->n_cen ? saved_hi : h->ctr_hi;
   DevCounters ctrs;
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
-  if (ctrs.bad_args) {  // a transport kernel's view of its own arguments disagreed
-    h->n_cen = 0;
-    h->n_new = 0;
+  // (the resident census was not touched, so it stays)
+  if (ctrs.bad_args)  // a transport kernel's view of its own arguments disagreed
     return fail(h, BH_ERR_DEVICE, "transport kernel argument check failed; nothing transported");
-  }
   for (uint32_t c = 0; c < h->n_cells; ++c) {
     if (abs_E) abs_E[c] += a[c];
     if (track_E) track_E[c] += t[c];
@@ -1535,18 +1588,7 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     stats->lattice = h->lattice ? 1 : 0;
     stats->tally = h->last_tally;
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
-    for (uint64_t i = 0; i < n; ++i) {
-      const uint8_t d = photons[i].descriptor[0];
-      if (d == BH_CENSUS) {
-        stats->n_census++;
-        stats->census_E += photons[i].E;
-      } else if (d == BH_EXIT) {
-        stats->n_exit++;
-        stats->exit_E += photons[i].E;
-      } else if (d == BH_KILLED) {
-        stats->n_killed++;
-      }
-    }
+    outcome_stats(photons, n, stats);
   }
   if (ctrs.stuck)
     return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
@@ -1567,8 +1609,8 @@ int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
   HIPCHK(h, hipSetDevice(h->device));
   if (stats) std::memset(stats, 0, sizeof *stats);
   if (n == 0) return BH_OK;
-  const uint64_t saved_hi = h->ctr_hi;
-  TRY(upload_aos(h, photons, n, h->C, 0));
+  uint64_t hi = 0;  // the photons' counter-high word; the resident census keeps its own
+  TRY(upload_aos(h, photons, n, h->C, 0, &hi));
   const uint32_t nc = h->n_cells;
   TRY(ensure(h, h->adj_g, nc));
   TRY(ensure(h, h->grad_sa, nc));
@@ -1583,7 +1625,7 @@ int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
   GlobalTally<true> sink{h->grad_sa.p, h->grad_f.p, nullptr, nullptr, 0, 0};
   k_adjoint<GlobalTally<true>><<<grid_for(n, 256), 256, 0, h->stream>>>(
-      dev_cells(h), h->C.soa, n, h->ctr_hi, h->adj_g.p, sink, h->d_ctrs);
+      dev_cells(h), h->C.soa, n, hi, h->adj_g.p, sink, h->d_ctrs);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
   std::vector<double> a(nc), f(nc);
@@ -1595,14 +1637,11 @@ int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
   HIPCHK(h, hipMemcpyAsync(hp + 256, h->d_ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->ctr_hi = h->n_cen ? saved_hi : h->ctr_hi;
   DevCounters ctrs;
   std::memcpy(&ctrs, hp + 256, sizeof ctrs);
-  if (ctrs.bad_args) {  // a transport kernel's view of its own arguments disagreed
-    h->n_cen = 0;
-    h->n_new = 0;
+  // (the resident census was not touched, so it stays)
+  if (ctrs.bad_args)  // a transport kernel's view of its own arguments disagreed
     return fail(h, BH_ERR_DEVICE, "transport kernel argument check failed; nothing transported");
-  }
   for (uint32_t c = 0; c < nc; ++c) {
     d_sigma_a[c] += a[c];
     d_fleck[c] += f[c];
