@@ -40,8 +40,8 @@ EXPORTS = [
     "bh_step_graph", "bh_get_tallies", "bh_census_size", "bh_census_export",
     "bh_photons_export",
     "bh_census_import",
-    "bh_transport_photons", "bh_adjoint_photons", "bh_rng_draws", "bh_threefry",
-    "bh_copy_probe",
+    "bh_transport_photons", "bh_transport_photons_noalloc", "bh_adjoint_photons",
+    "bh_rng_draws", "bh_threefry", "bh_copy_probe",
 ]
 
 
@@ -136,6 +136,7 @@ def lib() -> C.CDLL:
     L.bh_census_import.argtypes = [P, P, U64]
     L.bh_photons_export.argtypes = [P, P, U64, P]
     L.bh_transport_photons.argtypes = [P, P, U64, P, P, P]
+    L.bh_transport_photons_noalloc.argtypes = [P, P, U64, P, P, P]
     L.bh_adjoint_photons.argtypes = [P, P, U64, P, P, P, P]
     L.bh_rng_draws.argtypes = [P, U32, U64, U64, U32, P]
     L.bh_threefry.argtypes = [P, P, U64, P]
@@ -313,6 +314,16 @@ class Engine:
         st = StepStats()
         self._check(lib().bh_transport_photons(self._h, _p(photons), len(photons), _p(a),
                                                _p(t), C.byref(st)))
+        return a, t, st.as_dict()
+
+    def transport_photons_noalloc(self, photons: np.ndarray):
+        """transport_photons through the entry point that allocates no device
+        memory (its buffers come from the library's static scratch array)."""
+        assert photons.dtype == PHOTON_DTYPE and photons.flags.c_contiguous
+        a, t = np.zeros(self.n_cells), np.zeros(self.n_cells)
+        st = StepStats()
+        self._check(lib().bh_transport_photons_noalloc(self._h, _p(photons), len(photons),
+                                                       _p(a), _p(t), C.byref(st)))
         return a, t, st.as_dict()
 
     def adjoint_photons(self, photons: np.ndarray, abs_adjoint):

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -21,7 +22,39 @@
 
 using namespace bh;
 
+// Device memory of bh_transport_photons_noalloc: the photons' AOS staging
+// and SoA state, the tallies, counters, queue and FP32 replay lists of one
+// call, carved from this array instead of allocated.  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code the name
+// is only a host-side stand-in: the device address comes from
+// hipGetSymbolAddress, once per device (scratch_on_device), and only that
+// address is ever passed to a kernel or a copy.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+// Resolved on the first call for a device and kept.
+hipError_t scratch_on_device(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
@@ -179,6 +212,11 @@ struct bh_handle {
   double last_source_ms = 0.0;
   int n_cu = 256;
   size_t lds_budget = 48 * 1024;  // dynamic LDS a tally may use per block
+
+  // this device's address of `scratch` (bh_transport_photons_noalloc);
+  // null when it could not be resolved, with the reason
+  void* scratch = nullptr;
+  std::string scratch_err;
 };
 
 namespace {
@@ -752,6 +790,13 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
   HIPCHK(h, hipMalloc(&h->d_ctrs, sizeof(DevCounters)));
   HIPCHK(h, hipMalloc(&h->d_queue, sizeof(PersistentQueue)));
   HIPCHK(h, hipHostMalloc(&h->h_pinned, 4096, hipHostMallocDefault));
+  {  // the no-allocation entry point's device memory on this device
+    const hipError_t e = scratch_on_device(device, &h->scratch);
+    if (e != hipSuccess) {
+      h->scratch = nullptr;
+      h->scratch_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+    }
+  }
   if (h->opt.reserve_photons) {
     TRY(grow_store(h, h->W, h->opt.reserve_photons, 0));
     TRY(grow_store(h, h->C, h->opt.reserve_photons, 0));
@@ -1580,6 +1625,130 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     if (abs_E) abs_E[c] += a[c];
     if (track_E) track_E[c] += t[c];
   }
+  if (stats) {
+    stats->n_transported = n;
+    stats->n_segments = ctrs.segments;
+    stats->n_flushes = ctrs.flushes;
+    stats->n_replayed = ctrs.replays;
+    stats->lattice = h->lattice ? 1 : 0;
+    stats->tally = h->last_tally;
+    stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
+    outcome_stats(photons, n, stats);
+  }
+  if (ctrs.stuck)
+    return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
+                ctrs.stuck, kMaxSegments);
+  return BH_OK;
+}
+
+// The scratch layout of one bh_transport_photons_noalloc call; bytes = 0
+// when it does not fit.
+struct NoallocLayout {
+  size_t ctrs, queue, abs_E, track_E, aos, soa, rep_idx, rep_first, rep_E, bytes;
+};
+static NoallocLayout noalloc_layout(uint64_t n, uint32_t n_cells, bool fp32) {
+  NoallocLayout L{};
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    const size_t at = o;
+    o += align_up(b);
+    return at;
+  };
+  L.ctrs = take(sizeof(DevCounters));
+  L.queue = take(sizeof(PersistentQueue));
+  L.abs_E = take((size_t)n_cells * sizeof(double));
+  L.track_E = take((size_t)n_cells * sizeof(double));
+  L.aos = take((size_t)n * sizeof(AosPhoton));
+  L.soa = take(PhotonStore::bytes_for(n));
+  if (fp32) {
+    L.rep_idx = take((size_t)n * sizeof(uint64_t));
+    L.rep_first = take((size_t)n * sizeof(uint32_t));
+    L.rep_E = take((size_t)n * sizeof(double));
+  }
+  L.bytes = o;
+  return L;
+}
+
+int bh_transport_photons_noalloc(bh_handle* h, bh_photon* photons, uint64_t n, double* abs_E,
+                                 double* track_E, bh_step_stats* stats) {
+  if (!h || !h->stream || (n && !photons)) return BH_ERR_ARG;
+  if (!h->has_mesh)
+    return fail(h, BH_ERR_STATE, "bh_transport_photons_noalloc before bh_set_mesh");
+  if (!h->scratch)
+    return fail(h, BH_ERR_DEVICE, "no scratch memory on device %d: %s", h->device,
+                h->scratch_err.c_str());
+  HIPCHK(h, hipSetDevice(h->device));
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (n == 0) return BH_OK;
+  const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
+  const NoallocLayout L = noalloc_layout(n, h->n_cells, fp32);
+  if (L.bytes > kScratchBytes) {
+    const NoallocLayout one = noalloc_layout(1, h->n_cells, fp32);
+    const NoallocLayout two = noalloc_layout(2, h->n_cells, fp32);
+    const size_t per = two.bytes > one.bytes ? two.bytes - one.bytes : 1;
+    return fail(h, BH_ERR_CAPACITY,
+                "%llu photons on %u cells need %zu bytes of device scratch; it holds %zu "
+                "(about %llu photons on this mesh)",
+                (unsigned long long)n, h->n_cells, L.bytes, kScratchBytes,
+                (unsigned long long)(kScratchBytes > one.bytes ? (kScratchBytes - one.bytes) / per
+                                                               : 0));
+  }
+  uint64_t hi = 0;
+  TRY(check_aos(h, photons, n, &hi));
+  char* base = static_cast<char*>(h->scratch);
+  Sinks sk;
+  sk.abs_E = reinterpret_cast<double*>(base + L.abs_E);
+  sk.track_E = reinterpret_cast<double*>(base + L.track_E);
+  sk.ctrs = reinterpret_cast<DevCounters*>(base + L.ctrs);
+  sk.queue = reinterpret_cast<PersistentQueue*>(base + L.queue);
+  sk.rl = fp32 ? ReplayList{reinterpret_cast<uint64_t*>(base + L.rep_idx),
+                            reinterpret_cast<uint32_t*>(base + L.rep_first),
+                            reinterpret_cast<double*>(base + L.rep_E)}
+               : ReplayList{nullptr, nullptr, nullptr};
+  sk.ctr_hi = hi;
+  sk.additive = true;  // the caller's tallies are added to on the device
+  AosPhoton* aos = reinterpret_cast<AosPhoton*>(base + L.aos);
+  const PhotonSoA soa = PhotonStore::carve(base + L.soa, n);
+  const uint32_t nc = h->n_cells;
+  // the previous call's contents are still there: clear what starts at zero
+  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(sk.ctrs),
+                                      sizeof(DevCounters) / 8);
+  if (abs_E)
+    HIPCHK(h, hipMemcpyAsync(sk.abs_E, abs_E, nc * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
+  else
+    k_fill_f64<<<grid_for(nc, 256), 256, 0, h->stream>>>(sk.abs_E, nc, 0.0);
+  if (track_E)
+    HIPCHK(h, hipMemcpyAsync(sk.track_E, track_E, nc * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
+  else
+    k_fill_f64<<<grid_for(nc, 256), 256, 0, h->stream>>>(sk.track_E, nc, 0.0);
+  HIPCHK(h, hipMemcpyAsync(aos, photons, n * sizeof(AosPhoton), hipMemcpyHostToDevice,
+                           h->stream));
+  k_aos_to_soa<<<grid_for(n, 256), 256, 0, h->stream>>>(aos, n, soa, 0);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+  CensusOut in_place{};  // the caller's photons come back in place, in order
+  TRY(launch_transport(h, soa, n, nullptr, in_place, sk));
+  HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+  k_soa_to_aos<<<grid_for(n, 256), 256, 0, h->stream>>>(soa, 0, n, hi, dev_cells(h), aos, 0, 0);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(photons, aos, n * sizeof(AosPhoton), hipMemcpyDeviceToHost,
+                           h->stream));
+  if (abs_E)
+    HIPCHK(h, hipMemcpyAsync(abs_E, sk.abs_E, nc * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+  if (track_E)
+    HIPCHK(h, hipMemcpyAsync(track_E, sk.track_E, nc * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+  char* hp = static_cast<char*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp + 256, sk.ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  DevCounters ctrs;
+  std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  if (ctrs.bad_args)  // a transport kernel's view of its own arguments disagreed
+    return fail(h, BH_ERR_DEVICE, "transport kernel argument check failed; nothing transported");
   if (stats) {
     stats->n_transported = n;
     stats->n_segments = ctrs.segments;

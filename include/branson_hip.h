@@ -256,6 +256,22 @@ int bh_census_import(bh_handle *h, const bh_photon *in, uint64_t n);
 int bh_transport_photons(bh_handle *h, bh_photon *photons, uint64_t n,
                          double *abs_E, double *track_E, bh_step_stats *stats);
 
+/* bh_transport_photons without any device allocation (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments and results as bh_transport_photons, except that the
+ * tallies always use a form that adds to abs_E / track_E on the device
+ * (events identical, tallies equal to round-off).  All of the call's device
+ * memory (AOS staging, SoA state, tallies, counters, replay lists) comes
+ * from one static 64 MiB array per GPU, held until the process ends: about
+ * 270,000 photons per call on a 40^3 mesh; a call that needs more fails with
+ * BH_ERR_CAPACITY and the limit in bh_last_error.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (two handles on different
+ * threads or streams) race and must be serialised by the caller. */
+int bh_transport_photons_noalloc(bh_handle *h, bh_photon *photons, uint64_t n,
+                                 double *abs_E, double *track_E,
+                                 bh_step_stats *stats);
+
 /* Path-replay adjoint of the absorbed-energy tally (no counterpart in the
  * reference).  For photons in their state BEFORE transport (the input of
  * bh_transport_photons), the cells of the last bh_set_mesh and
