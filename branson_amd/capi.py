@@ -71,7 +71,8 @@ class StepStats(C.Structure):
                 ("n_exit", C.c_uint64), ("n_killed", C.c_uint64), ("n_segments", C.c_uint64),
                 ("n_flushes", C.c_uint64), ("transport_ms", C.c_double),
                 ("census_ms", C.c_double), ("source_ms", C.c_double),
-                ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("tally", C.c_int32)]
+                ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("tally", C.c_int32),
+                ("replay_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -425,7 +426,8 @@ class DriverReport(C.Structure):
                 ("source_ms", C.c_double), ("transport_ms", C.c_double),
                 ("census_ms", C.c_double), ("prepare_ms", C.c_double),
                 ("device_phase_ms", C.c_double), ("finish_ms", C.c_double),
-                ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("tally", C.c_int32)]
+                ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("tally", C.c_int32),
+                ("replay_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -209,6 +209,7 @@ struct bh_handle {
   std::vector<double> em_host, src_host;  // the step's energies, for exact counts
 
   hipEvent_t ev[6] = {};
+  bool replay_timed = false;  // ev[3] marks the start of the last FP64 replay
   double last_source_ms = 0.0;
   int n_cu = 256;
   size_t lds_budget = 48 * 1024;  // dynamic LDS a tally may use per block
@@ -520,6 +521,9 @@ bool use_lds_tally(const bh_handle* h, bool persistent) {
 template <class T>
 int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const ReplayList& rl,
                   T t, size_t shm, const CensusOut& co, const StepParams* P, const Sinks& s) {
+  // the replay's share of the transport time (not inside a captured graph)
+  h->replay_timed = P == nullptr;
+  if (h->replay_timed) HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
   k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(W, co, cells, rl,
                                                                      s.ctr_hi, t, s.ctrs, P);
   HIPCHK(h, hipGetLastError());
@@ -1155,6 +1159,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   // restored from the transport indices afterwards
   CensusOut co;
   TRY(census_prepare(h, n, next_dt, co));
+  h->replay_timed = false;
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
   TRY(launch_transport(h, h->W.soa, n, nullptr, co, handle_sinks(h, h->ctr_hi)));
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
@@ -1209,6 +1214,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     stats->census_ms = elapsed_ms(h->ev[1], h->ev[2]);
     stats->source_ms = h->last_source_ms;
+    stats->replay_ms = h->replay_timed ? elapsed_ms(h->ev[3], h->ev[1]) : 0.0;
   }
   if (ctrs.stuck)
     return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
@@ -1971,6 +1977,7 @@ int bh_set_material(bh_handle* h, const bh_material* m, double replicated_factor
   if (h->n_inst) return fail(h, BH_ERR_STATE, "resident step physics runs one problem");
   if (n_ranks == 0 || rank >= n_ranks || !(replicated_factor > 0.0))
     return fail(h, BH_ERR_ARG, "need n_ranks >= 1, rank < n_ranks, replicated_factor > 0");
+  if (n_ranks > kMaxRanks) return fail(h, BH_ERR_ARG, "at most %u ranks", kMaxRanks);
   const double* cols[9] = {m->opA, m->opB, m->opC, m->opS, m->cv, m->rho, m->T_e, m->T_r, m->T_s};
   for (const double* c : cols)
     if (!c) return fail(h, BH_ERR_ARG, "every bh_material array is required");
@@ -2066,6 +2073,9 @@ int bh_step_resident(bh_handle* h, uint64_t n_user_photons, uint32_t seed, uint3
                                               h->mat_ranks, 0, h->d_phys, h->d_inst2);
     HIPCHK(h, hipGetLastError());
     if (h->mat_ranks > 1) {  // with one rank the reassignment leaves every energy as it is
+      k_phys_rank_sums<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, n,
+                                                   n_user_photons, h->mat_ranks, h->mat_rf,
+                                                   first ? 1 : 0, h->d_phys, h->d_inst2);
       k_phys_reassign<<<grid_for(n, 256), 256, 0, h->stream>>>(
           h->em.p, h->src.p, h->cen.p, n, n_user_photons, h->mat_ranks, h->mat_rank, h->mat_rf,
           first ? 1 : 0, h->d_phys);

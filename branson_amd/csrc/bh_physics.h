@@ -36,6 +36,8 @@ struct PhysTotals {
   double absorbed, post_mat;                    // after update_temperature
   double pre_census;                            // step 1: initial census energy
   double global_E;                              // E_tot of the photon counts
+  double reassign_E;                            // several ranks: the all-reduced
+                                                // step total the reassignment tests
   double E_lo, E_hi;                            // bounds on the serial-order E_tot
   uint32_t ambiguous;                           // a count could differ within them
   uint32_t pad;
@@ -231,11 +233,12 @@ __global__ __launch_bounds__(256) void k_phys_cells(CellGeo* __restrict__ geo,
 // in order with the same (uniform) index: scalar loads, one dependent chain
 // of FP64 adds per sum.
 //
-// Pass 1 (pass = 0): the sums of calculate_photon_energy's loop, E_tot for the
-// photon counts -- this rank's total with one rank; with several, n_ranks
-// times the local sum, the small-energy reassignment (k_phys_reassign) and
-// pass 2 (pass = 1), which recomputes the step totals only -- and the source
-// parameter records' E_tot.
+// Pass 1 (pass = 0): the sums of calculate_photon_energy's loop; E_tot for
+// the photon counts with one rank (its own total, the all-reduce of one
+// value); with several, the all-reduced step total of the small-energy
+// reassignment (n_ranks equal values summed in rank order), after which
+// k_phys_rank_sums sets E_tot and k_phys_reassign applies this rank's
+// share.  Pass 2 (pass = 1) recomputes this rank's step totals only.
 __global__ __launch_bounds__(kWave) void k_phys_totals(const double* __restrict__ em,
                                                        const double* __restrict__ src,
                                                        const double* __restrict__ cen,
@@ -264,19 +267,70 @@ __global__ __launch_bounds__(kWave) void k_phys_totals(const double* __restrict_
   out->tot_source = t_src;
   out->total_photon_E = total;
   if (pass == 0) {
-    out->global_E = (n_ranks == 1) ? total : (double)n_ranks * (t_em + t_cen + t_src);
+    const double local3 = t_em + t_cen + t_src;
+    double g = local3;
+    for (uint32_t r = 1; r < n_ranks; ++r) g += local3;
+    out->reassign_E = g;
     out->pre_mat = pre_mat;
     out->e_sum = e1;
     out->e_sq = e2;
-    inst2[0].total_E = out->global_E;
-    inst2[1].total_E = out->global_E;
+    if (n_ranks == 1) {
+      out->global_E = total;
+      inst2[0].total_E = total;
+      inst2[1].total_E = total;
+    }
   }
 }
 
-// Replicated small-energy reassignment (src/mesh.h:291-315), more than one
-// rank: a cell whose energy would get no photon of its own from the global
-// n_user (int(n_user * E / global_E) == 0) is given whole to rank
-// cell % n_ranks.
+// Small-energy reassignment (src/mesh.h:291-307) of one cell's energies as
+// rank r of n_ranks sees them: an energy the all-rank source would give no
+// photon of its own (int(n_user * (E / reassign_E)) == 0) goes whole to
+// rank cell % n_ranks, and is zero on the others.
+__device__ __forceinline__ void reassign_cell(uint32_t c, uint32_t r, uint32_t n_ranks,
+                                              double rf, double nu, double g, int first_step,
+                                              double& e, double& s, double& ce) {
+  const bool mine = (c % n_ranks) == r;
+  if (first_step && ce > 0.0 && (int)(nu * (ce / g)) == 0) ce = mine ? ce / rf : 0.0;
+  if (e > 0.0 && (int)(nu * (e / g)) == 0) e = mine ? e / rf : 0.0;
+  if (s > 0.0 && (int)(nu * (s / g)) == 0) s = mine ? s / rf : 0.0;
+}
+
+// More than one rank, before k_phys_reassign: E_tot of the photon counts is
+// the all-reduce of every rank's total_photon_E after its own reassignment
+// (src/replicated_driver.h:57-59), summed in rank order.  Lane r of the one
+// wave computes rank r's total in the reference's cell order from the
+// shared pre-reassignment energies.
+constexpr uint32_t kMaxRanks = 1024;
+__global__ __launch_bounds__(kWave) void k_phys_rank_sums(const double* __restrict__ em,
+                                                          const double* __restrict__ src,
+                                                          const double* __restrict__ cen,
+                                                          uint32_t n, uint64_t n_user,
+                                                          uint32_t n_ranks, double rf,
+                                                          int first_step,
+                                                          PhysTotals* __restrict__ out,
+                                                          SourceInst* __restrict__ inst2) {
+  __shared__ double rank_total[kMaxRanks];
+  const double g = out->reassign_E;
+  const double nu = (double)(uint32_t)n_user;
+  for (uint32_t r = threadIdx.x; r < n_ranks && r < kMaxRanks; r += kWave) {
+    double t = 0.0;
+    for (uint32_t c = 0; c < n; ++c) {
+      double e = em[c], s = src[c], ce = cen[c];
+      reassign_cell(c, r, n_ranks, rf, nu, g, first_step, e, s, ce);
+      t += s + ce + e;
+    }
+    rank_total[r] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double all = 0.0;
+  for (uint32_t r = 0; r < n_ranks && r < kMaxRanks; ++r) all += rank_total[r];
+  out->global_E = all;
+  inst2[0].total_E = all;
+  inst2[1].total_E = all;
+}
+
+// This rank's share of the reassignment, in place.
 __global__ __launch_bounds__(256) void k_phys_reassign(double* __restrict__ em,
                                                        double* __restrict__ src,
                                                        double* __restrict__ cen, uint32_t n,
@@ -285,13 +339,12 @@ __global__ __launch_bounds__(256) void k_phys_reassign(double* __restrict__ em,
                                                        const PhysTotals* __restrict__ tot) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const double global_E = tot->global_E;
-  const double nu = (double)(uint32_t)n_user;
-  const bool mine = (c % n_ranks) == rank;
-  const double ce = cen[c], e = em[c], s = src[c];
-  if (first_step && ce > 0.0 && (int)(nu * (ce / global_E)) == 0) cen[c] = mine ? ce / rf : 0.0;
-  if (e > 0.0 && (int)(nu * (e / global_E)) == 0) em[c] = mine ? e / rf : 0.0;
-  if (s > 0.0 && (int)(nu * (s / global_E)) == 0) src[c] = mine ? s / rf : 0.0;
+  double e = em[c], s = src[c], ce = cen[c];
+  reassign_cell(c, rank, n_ranks, rf, (double)(uint32_t)n_user, tot->reassign_E, first_step, e,
+                s, ce);
+  cen[c] = ce;
+  em[c] = e;
+  src[c] = s;
 }
 
 // Mesh::update_temperature, one rank (src/mesh.h:327-423): new electron

@@ -363,27 +363,44 @@ void calculate_photon_energy(bh_driver* d) {
     tot_source += m.source_E[i];
     m.total_photon_E += m.source_E[i] + m.census_E[i] + m.emission_E[i];
   }
-  // replicated small-energy reassignment (src/mesh.h:291-315): every rank
-  // holds the same local sums, so the all-rank sum is n_ranks times them
-  const double global_E = (double)n_ranks * (tot_emission + tot_census + tot_source);
-  const double n_user = (double)(uint32_t)d->deck.n_photons;  // n_user_photons is uint32 here
+  // replicated small-energy reassignment (src/mesh.h:287-315).  The
+  // all-reduce of the three step totals sums n_ranks equal values, taken in
+  // rank order; a cell energy the all-rank source would give no photon of
+  // its own goes whole to rank cell % n_ranks.
+  const double local3 = tot_emission + tot_census + tot_source;
+  double global_E = local3;
+  for (uint32_t r = 1; r < n_ranks; ++r) global_E += local3;
+  const uint32_t n_user = (uint32_t)d->deck.n_photons;  // n_user_photons is uint32 here
+  auto reassign = [&](uint32_t i, uint32_t r, double& em, double& src, double& cen) {
+    const bool mine = (i % n_ranks) == r;
+    if (s.step == 1 && cen > 0.0 && int(n_user * (cen / global_E)) == 0)
+      cen = mine ? cen / rf : 0.0;
+    if (em > 0.0 && int(n_user * (em / global_E)) == 0) em = mine ? em / rf : 0.0;
+    if (src > 0.0 && int(n_user * (src / global_E)) == 0) src = mine ? src / rf : 0.0;
+  };
+  // the source's E_tot (src/replicated_driver.h:57-59): the all-reduce of
+  // every rank's total after its own reassignment, in rank order
+  double all_ranks_E = 0.0;
+  for (uint32_t r = 0; r < n_ranks && n_ranks > 1; ++r) {
+    double t = 0.0;
+    for (uint32_t i = 0; i < m.n; ++i) {
+      double em = m.emission_E[i], src = m.source_E[i], cen = m.census_E[i];
+      reassign(i, r, em, src, cen);
+      t += src + cen + em;
+    }
+    all_ranks_E += t;
+  }
   tot_census = tot_emission = tot_source = 0;
   m.total_photon_E = 0;
   for (uint32_t i = 0; i < m.n; ++i) {
-    if (s.step == 1 && m.census_E[i] > 0.0 && int(n_user * (m.census_E[i] / global_E)) == 0)
-      m.census_E[i] = (i % n_ranks == rank) ? m.census_E[i] / rf : 0.0;
-    if (m.emission_E[i] > 0.0 && int(n_user * (m.emission_E[i] / global_E)) == 0)
-      m.emission_E[i] = (i % n_ranks == rank) ? m.emission_E[i] / rf : 0.0;
-    if (m.source_E[i] > 0.0 && int(n_user * (m.source_E[i] / global_E)) == 0)
-      m.source_E[i] = (i % n_ranks == rank) ? m.source_E[i] / rf : 0.0;
+    reassign(i, rank, m.emission_E[i], m.source_E[i], m.census_E[i]);
     tot_emission += m.emission_E[i];
     tot_census += m.census_E[i];
     tot_source += m.source_E[i];
     m.total_photon_E += m.source_E[i] + m.census_E[i] + m.emission_E[i];
   }
-  // one rank: its own sum, as the reference's all-reduce of one value;
-  // a rank's share of n: every cell's full energy once (to round-off)
-  m.global_photon_E = (n_ranks == 1) ? m.total_photon_E : global_E;
+  // one rank: the all-reduce of its own total
+  m.global_photon_E = (n_ranks == 1) ? m.total_photon_E : all_ranks_E;
   s.pre_mat_E = pre_mat;
   s.emission_E = tot_emission;
   s.source_E = tot_source;
@@ -627,6 +644,7 @@ int bh_driver_finish(bh_driver* d, bh_driver_report* rep) {
   R.source_ms = d->stats.source_ms;
   R.transport_ms = d->stats.transport_ms;
   R.census_ms = d->stats.census_ms;
+  R.replay_ms = d->stats.replay_ms;
   R.prepare_ms = d->prepare_ms;
   R.device_phase_ms = d->device_ms;
   d->prev_census_E = d->stats.census_E;
@@ -683,7 +701,7 @@ int bh_driver_get_physics(const bh_driver* d, double* sigma_a, double* sigma_s, 
     if (rc != BH_OK) return rc;
     if (census_E) std::memset(census_E, 0, n * sizeof(double));
     if (source_face) std::memcpy(source_face, m.source_face.data(), n);
-    if (total_E) *total_E = 0.0;
+    if (total_E) *total_E = d->rr.total_E;  // E_tot of the last resident step
     return BH_OK;
   }
   if (sigma_a) std::memcpy(sigma_a, m.sigma_a.data(), n * m.G * sizeof(double));

@@ -128,6 +128,8 @@ typedef struct bh_step_stats {
   int32_t tally;          /* tally form of the last transport: 1 one global
                              atomic per flush, 2 block-private LDS, 3 two
                              global atomics per flush, 4 wave hash table   */
+  double replay_ms;       /* FP32 tracking: device time of the FP64 replay,
+                             included in transport_ms (0 in FP64)          */
 } bh_step_stats;
 
 /* A photon in the reference's 120-byte AOS layout (src/photon.h:172-181). */

@@ -53,6 +53,7 @@ typedef struct bh_driver_report {
   uint64_t n_replayed;   /* FP32 tracking: histories finished in FP64       */
   int32_t lattice;       /* 1: tracked on the lattice records               */
   int32_t tally;         /* tally form (bh_step_stats.tally)                */
+  double replay_ms;      /* FP32 tracking: FP64 replay part of transport_ms */
 } bh_driver_report;
 
 typedef struct bh_driver bh_driver;
@@ -89,8 +90,8 @@ int bh_driver_get_temperature(const bh_driver *d, double *T_e, double *T_r);
 int bh_driver_get_mesh(const bh_driver *d, double *nodes, int32_t *bc, uint32_t *next,
                        uint32_t *region);
 /* Cell data and source energies of the step last prepared (with
- * device_physics: of the step last transported; census_E and total_E are
- * then not kept and come back as 0). */
+ * device_physics: of the step last transported; census_E is then not kept
+ * and comes back as 0, total_E is that step's E_tot). */
 int bh_driver_get_physics(const bh_driver *d, double *sigma_a, double *sigma_s, double *fleck,
                           double *emission_E, double *source_E, double *census_E,
                           int8_t *source_face, double *total_E);

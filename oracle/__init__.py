@@ -169,6 +169,7 @@ def lib() -> C.CDLL:
         L.orc_sim_create.argtypes = [P]
         L.orc_sim_create.restype = P
         L.orc_sim_destroy.argtypes = [P]
+        L.orc_sim_set_rank.argtypes = [P, U32, U32]
         L.orc_sim_n_cells.argtypes = [P]
         L.orc_sim_n_cells.restype = U32
         L.orc_sim_finished.argtypes = [P]
@@ -486,6 +487,12 @@ class Sim:
         if h:
             lib().orc_sim_destroy(h)
             self._h = None
+
+    def set_rank(self, rank: int, n_ranks: int):
+        """Run as replicated rank `rank` of `n_ranks` (src/mesh.h:80-87,
+        287-315): a 1/n_ranks share of every energy, the small-energy
+        reassignment, the all-rank E_tot and the rank's stream offsets."""
+        lib().orc_sim_set_rank(self._h, rank, n_ranks)
 
     def finished(self) -> bool:
         return bool(lib().orc_sim_finished(self._h))

@@ -753,6 +753,10 @@ struct orc_sim {
   /* census list */
   orc_photon *census;
   uint64_t n_census;
+  /* replicated rank of this simulation (src/mesh.h:80-87) and the all-rank
+   * photon energy the source divides by (src/replicated_driver.h:57-59) */
+  uint32_t rank, n_ranks;
+  double global_photon_E;
   /* IMC_State (src/imc_state.h:42-45, 112-134, 292-296) */
   double time, dt, t_stop, t_mult, dt_max;
   uint32_t step;
@@ -907,7 +911,15 @@ orc_sim *orc_sim_create(const orc_deck *d) {
   s->t_mult = d->t_mult;
   s->dt_max = d->dt_max;
   s->step = 1;
+  s->rank = 0;
+  s->n_ranks = 1;
   return s;
+}
+
+void orc_sim_set_rank(orc_sim *s, uint32_t rank, uint32_t n_ranks) {
+  s->n_ranks = n_ranks ? n_ranks : 1;
+  s->rank = rank < s->n_ranks ? rank : 0;
+  s->prepared = 0;
 }
 
 void orc_sim_destroy(orc_sim *s) {
@@ -934,13 +946,37 @@ static double orc_face_area(const double *nd, int face) {
   return -1.0;
 }
 
-/* Mesh::calculate_photon_energy (src/mesh.h:237-323), one rank, so the
- * replicated factor is 1 and the small-energy reassignment keeps every
- * energy; its second pass still recomputes the totals in the same order. */
+/* One cell's three energies after the replicated small-energy reassignment
+ * (src/mesh.h:291-307) as rank r of n_ranks sees them: an energy that the
+ * all-rank source would give no photon of its own, int(n_user * (E /
+ * global_source_E)) == 0, goes whole (E / replicated_factor) to rank
+ * i % n_ranks and is zero on every other rank. */
+static void orc_reassign(uint32_t i, uint32_t r, uint32_t n_ranks, double rf,
+                         uint32_t n_user, double global_source_E, uint32_t step,
+                         double *em, double *src, double *cen) {
+  const int mine = (i % n_ranks) == r;
+  if (step == 1 && *cen > 0.0 && (int)(n_user * (*cen / global_source_E)) == 0)
+    *cen = mine ? *cen / rf : 0.0;
+  if (*em > 0.0 && (int)(n_user * (*em / global_source_E)) == 0)
+    *em = mine ? *em / rf : 0.0;
+  if (*src > 0.0 && (int)(n_user * (*src / global_source_E)) == 0)
+    *src = mine ? *src / rf : 0.0;
+}
+
+/* Mesh::calculate_photon_energy (src/mesh.h:237-323) for rank `rank` of
+ * `n_ranks` (replicated_factor = 1 / n_ranks, src/mesh.h:87).  The two
+ * MPI_Allreduce(SUM) calls -- of the three step totals inside
+ * calculate_photon_energy (src/mesh.h:292-294) and of total_photon_E in the
+ * driver (src/replicated_driver.h:57-59) -- are modelled as sums taken in
+ * rank order.  Every rank holds the same mesh, so the first sums n_ranks
+ * copies of this rank's value; for the second, every rank's reassigned
+ * total is computed here from the same per-cell energies.  One rank: both
+ * are this rank's own values and the reassignment keeps every energy. */
 int orc_sim_prepare(orc_sim *s) {
   const double dt = s->dt;
   const uint32_t step = s->step;
-  const double rf = 1.0;
+  const double rf = 1.0 / (double)s->n_ranks;
+  const uint32_t n_user = (uint32_t)s->n_user; /* n_user_photons is uint32_t */
   double tot_census = 0.0, tot_emission = 0.0, tot_source = 0.0;
   double pre_mat = 0.0;
   s->total_photon_E = 0.0;
@@ -975,14 +1011,33 @@ int orc_sim_prepare(orc_sim *s) {
     tot_source += s->source_E[i];
     s->total_photon_E += s->source_E[i] + s->census_E[i] + s->emission_E[i];
   }
+  /* src/mesh.h:289-315 (replicated is set for every replicated run) */
+  const double local3 = tot_emission + tot_census + tot_source;
+  double global_source_E = local3;
+  for (uint32_t r = 1; r < s->n_ranks; ++r) global_source_E += local3;
+  /* every rank's total after its own reassignment, summed in rank order;
+   * this rank's is recomputed below exactly as the reference's loop does */
+  double global_photon_E = 0.0;
+  for (uint32_t r = 0; r < s->n_ranks && s->n_ranks > 1; ++r) {
+    double t = 0.0;
+    for (uint32_t i = 0; i < s->n_cells; ++i) {
+      double em = s->emission_E[i], src = s->source_E[i], cen = s->census_E[i];
+      orc_reassign(i, r, s->n_ranks, rf, n_user, global_source_E, step, &em, &src, &cen);
+      t += src + cen + em;
+    }
+    global_photon_E += t;
+  }
   tot_census = tot_emission = tot_source = 0.0;
   s->total_photon_E = 0.0;
   for (uint32_t i = 0; i < s->n_cells; ++i) {
+    orc_reassign(i, s->rank, s->n_ranks, rf, n_user, global_source_E, step,
+                 &s->emission_E[i], &s->source_E[i], &s->census_E[i]);
     tot_emission += s->emission_E[i];
     tot_census += s->census_E[i];
     tot_source += s->source_E[i];
     s->total_photon_E += s->source_E[i] + s->census_E[i] + s->emission_E[i];
   }
+  s->global_photon_E = (s->n_ranks > 1) ? global_photon_E : s->total_photon_E;
   s->pre_mat_E = pre_mat;
   s->emission_tot = tot_emission;
   s->source_tot = tot_source;
@@ -1003,7 +1058,7 @@ int orc_sim_step(orc_sim *s, int n_threads, orc_step_report *rep) {
   memset(rep, 0, sizeof *rep);
   if (!s->prepared) orc_sim_prepare(s);
   s->prepared = 0;
-  const double global_E = s->total_photon_E;
+  const double global_E = s->global_photon_E;
   const double dt = s->dt;
   orc_cells cells = {s->n_cells, s->G, s->nodes, s->bc, s->next,
                      s->sigma_a, s->sigma_s, s->fleck};
@@ -1012,23 +1067,23 @@ int orc_sim_step(orc_sim *s, int n_threads, orc_step_report *rep) {
   const double ts = orc_seconds();
   if (s->step == 1) {
     const uint64_t nc = orc_make_initial_census(&cells, s->census_E, s->n_user,
-                                                global_E, s->seed, 0, dt, NULL,
-                                                0);
+                                                global_E, s->seed, s->rank, dt,
+                                                NULL, 0);
     free(s->census);
     s->census = (orc_photon *)malloc(sizeof(orc_photon) * (nc ? nc : 1));
     orc_make_initial_census(&cells, s->census_E, s->n_user, global_E, s->seed,
-                            0, dt, s->census, nc);
+                            s->rank, dt, s->census, nc);
     s->n_census = nc;
   }
   s->pre_census_E = orc_list_E(s->census, s->n_census);
 
   const uint64_t n_new = orc_make_photons(&cells, s->emission_E, s->source_E,
                                           s->source_face, s->n_user, global_E,
-                                          s->seed, s->step, 0, dt, NULL, 0);
+                                          s->seed, s->step, s->rank, dt, NULL, 0);
   const uint64_t n_all = n_new + s->n_census;
   orc_photon *all = (orc_photon *)malloc(sizeof(orc_photon) * (n_all ? n_all : 1));
   orc_make_photons(&cells, s->emission_E, s->source_E, s->source_face,
-                   s->n_user, global_E, s->seed, s->step, 0, dt, all, n_new);
+                   s->n_user, global_E, s->seed, s->step, s->rank, dt, all, n_new);
   if (s->n_census)
     memcpy(all + n_new, s->census, sizeof(orc_photon) * s->n_census);
   rep->transported = n_all;
@@ -1111,7 +1166,7 @@ void orc_sim_get_physics(const orc_sim *s, double *sigma_a, double *sigma_s,
   if (source_E) memcpy(source_E, s->source_E, sizeof(double) * n);
   if (census_E) memcpy(census_E, s->census_E, sizeof(double) * n);
   if (source_face) memcpy(source_face, s->source_face, n);
-  if (total_E) *total_E = s->total_photon_E;
+  if (total_E) *total_E = s->global_photon_E;
 }
 
 void orc_sim_get_tallies(const orc_sim *s, double *abs_E, double *track_E) {

@@ -12,21 +12,39 @@
  * double with contraction disabled (-ffp-contract=off); transcendental calls
  * go to the host libm exactly as the reference's do.
  *
- * Pinning (what fixes each function independently of this file):
+ * Pinning (what fixes each function independently of this file; the tests
+ * named are in tests/):
  *   - orc_threefry2x64_20: known-answer vectors dumped from the reference's own
  *     vendored Random123 header compiled in place (oracle/_ref), plus the
- *     published all-zero Random123 vector.
+ *     published all-zero Random123 vector (test_oracle_rng.py).
  *   - orc_u01: the closed form documented at src/RNG.h:189-201 (odd multiples
  *     of 2^-53; extremes 2^-53 and 1-2^-53).
  *   - orc_distance_to_boundary / move / reflect: closed forms of
  *     src/test/test_cell.cc:79-133 and src/test/test_photon.cc:69-126.
- *   - samplers: statistical means of src/test/test_sampling_functions.cc and
- *     closed forms (unit direction norm, positions inside the cell).
+ *   - samplers and make_photons: source photons hand traced from the pinned
+ *     draws (positions, directions, lives, groups, draw counts, streams),
+ *     statistical means, unit norms, cell containment; the group walk as
+ *     inversion of its CDF (test_oracle_pins.py).
  *   - source counts: the closed form max(1, int(n*E/E_tot)) (src/source.h).
- *   - transport + census + step physics, end to end: per-step photon counts,
- *     census sizes and energies printed by the reference executable for the
- *     100-cell Marshak slab (10 steps, seed 777) and the 40^3 cube
- *     (1 step, seed 1234); conservation to round-off.
+ *   - transport: effective scattering in an infinite medium (scatter rate
+ *     sigma_eff = (1-f) sigma_a + sigma_s, re-sampling fraction
+ *     (1-f) sigma_a / sigma_eff, weight exp(-sigma_a f L), tallies, every
+ *     draw accounted for, isotropy), hand-traced single-scatter histories,
+ *     the kill deposit, and the cosine-law beam through a pure absorber
+ *     (2 E3(tau)).
+ *   - post_process: census order, census life and totals on a hand-built
+ *     list.
+ *   - step physics: the textbook implicit Monte Carlo forms (Fleck &
+ *     Cummings 1971) over three steps of a two-region deck with
+ *     temperature-dependent opacities; the material energy balance and the
+ *     track-length radiation temperature.
+ *   - rank share: n ranks' energies against the one-rank energies (every
+ *     cell's energy summed over ranks; small energies whole on rank
+ *     cell % n); every rank's E_tot equal to the one-rank total.
+ *   - whole steps: per-step photon counts, census sizes and energies printed
+ *     by the reference executable for the 100-cell Marshak slab (10 steps,
+ *     seed 777) and the 40^3 cube (1 step, seed 1234); conservation to
+ *     round-off.
  */
 #ifndef BRANSON_ORACLE_H
 #define BRANSON_ORACLE_H
@@ -187,6 +205,13 @@ typedef struct orc_step_report {
 
 orc_sim *orc_sim_create(const orc_deck *deck);
 void orc_sim_destroy(orc_sim *sim);
+/* Runs the simulation as rank `rank` of `n_ranks` replicated ranks
+ * (default 0 of 1): replicated_factor 1/n_ranks, the small-energy
+ * reassignment and the all-rank E_tot of src/mesh.h:287-315 and
+ * src/replicated_driver.h:57-59, and the rank's stream offsets.  A step
+ * transports this rank's share only; its temperature update then uses this
+ * rank's tallies, not the all-rank sum the reference reduces first. */
+void orc_sim_set_rank(orc_sim *sim, uint32_t rank, uint32_t n_ranks);
 uint32_t orc_sim_n_cells(const orc_sim *sim);
 int orc_sim_finished(const orc_sim *sim);
 /* one replicated time step (src/replicated_driver.h:47-121) */
@@ -195,7 +220,8 @@ int orc_sim_step(orc_sim *sim, int n_threads, orc_step_report *rep);
 void orc_sim_get_mesh(const orc_sim *sim, double *nodes, int32_t *bc,
                       uint32_t *next, uint32_t *region);
 /* state after the last calculate_photon_energy: per-cell opacities, Fleck
- * factor and the three source energies of the current step */
+ * factor and the three source energies of the current step; total_E is the
+ * all-rank photon energy the source divides by */
 void orc_sim_get_physics(const orc_sim *sim, double *sigma_a, double *sigma_s,
                          double *fleck, double *emission_E, double *source_E,
                          double *census_E, int8_t *source_face,

@@ -145,3 +145,36 @@ def test_resident_engine_entry_points():
             pre = st["census_E"]
         T_e, T_r = e.temperature()
     assert np.all(np.isfinite(T_e)) and T_e[face == 0].mean() > T_e[face != 0].mean()
+
+
+@pytest.mark.parametrize("deck,kw", [
+    ("marshak_slab_100", dict(n_ranks=2, rank=1)),
+    ("marshak_slab_100", dict(n_ranks=3, rank=2)),
+    ("cube_40", dict(n_ranks=4, rank=3, n_photons=20_000)),
+], ids=["marshak-1of2", "marshak-2of3", "cube-sparse-3of4"])
+def test_resident_rank_share_matches_oracle(deck, kw):
+    """A replicated rank's first step with its physics on the device
+    (small-energy reassignment and all-rank E_tot, src/mesh.h:287-315,
+    src/replicated_driver.h:57-59) against the oracle's restatement for the
+    same rank: per-cell energies to the last bit of pow (1e-13), E_tot to
+    its summation order, photon counts, census size and energies."""
+    import oracle
+    path = os.path.join(GOLDEN, "decks", deck + ".xml")
+    with capi.Driver(path, algorithm=capi.ALGO_PERSISTENT, device_physics=True, **kw) as d:
+        rep = d.step()
+        ph = d.physics()
+    od = oracle.Deck.from_xml(path)
+    if "n_photons" in kw:
+        od.n_photons = kw["n_photons"]
+    sim = oracle.Sim(od)
+    sim.set_rank(kw["rank"], kw["n_ranks"])
+    sim.prepare()
+    oph = sim.physics()
+    orep = sim.step(8)
+    for k in ("sigma_a", "fleck", "emission_E", "source_E"):
+        assert _rel(ph[k], oph[k]) <= 1e-13, k
+    assert _rel(ph["total_E"], oph["total_E"]) <= 1e-13
+    assert rep["transported"] == orep["transported"]
+    assert rep["census_size"] == orep["census_size"]
+    for k in ("emission_E", "source_E", "pre_census_E", "post_census_E", "exit_E"):
+        assert _rel(rep[k], orep[k]) <= 1e-12 or rep[k] == orep[k], k

@@ -105,3 +105,25 @@ def test_overrides_and_clock():
         assert d.n_photons == 12345
         assert d.seed == 1234
         assert not d.finished()
+
+
+@pytest.mark.parametrize("deck,rank,n_ranks,n_photons", [
+    (DECKS[0], 1, 2, 0), (DECKS[0], 2, 3, 0), (DECKS[1], 3, 4, 20_000), (DECKS[1], 0, 3, 0)],
+    ids=["marshak-1of2", "marshak-2of3", "cube-sparse-3of4", "cube-0of3"])
+def test_rank_share_physics_matches_oracle(deck, rank, n_ranks, n_photons):
+    """A replicated rank's share (src/mesh.h:287-315): this rank's per-cell
+    energies after the small-energy reassignment and the all-rank E_tot
+    bit for bit against the oracle's restatement for the same rank."""
+    with capi.Driver(deck, rank=rank, n_ranks=n_ranks, n_photons=n_photons) as d:
+        d.calculate_physics()
+        ph = d.physics()
+    od = oracle.Deck.from_xml(deck)
+    if n_photons:
+        od.n_photons = n_photons
+    sim = oracle.Sim(od)
+    sim.set_rank(rank, n_ranks)
+    sim.prepare()
+    oph = sim.physics()
+    for k in ("sigma_a", "sigma_s", "fleck", "emission_E", "source_E", "census_E"):
+        assert np.array_equal(ph[k], oph[k]), k
+    assert ph["total_E"] == oph["total_E"]
