@@ -1,29 +1,34 @@
 """Benchmark: replicated IMC transport phase on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-chosen]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--workload headline|cube|hohlraum|batched|adjoint] [--fp32-tol T]
 
-Workload: the plain gray 3D cube throughput configuration -- 40x40x40 cells of
-0.05 cm, vacuum faces, T = 1 keV, emission only, sigma_a = 5/cm,
-sigma_s = 1/cm, dt = 0.001 sh, 1e7 photons per step, seed 1234
-(branson_amd/decks/cube_40_throughput.xml, 5 steps by default).
+Default workload (headline): the 30-group hohlraum-like configuration -- 64^3
+cells of 0.01 cm, a 4-cell wall (sigma_a = 1000/cm, rho = 19) around a
+low-density fill (sigma_a = 0.1/cm, sigma_s = 0.01/cm, rho = 0.001), a 0.3
+keV source on the z = 0 face, 30 groups, 5e7 photons per step
+(branson_amd/decks/hohlraum_64.xml) -- in the plain FP64 form and in the
+FP32-tracking form (FP64 energies and tallies) at the per-cell tally error
+bound fp32_tol (default 1e-6).  The chosen form is the faster of the two whose
+step-1 per-cell absorbed-energy tallies are within 1e-6 (max relative over
+cells) of the FP64 form's; value is its histories per second.  The 40^3 cube
+throughput configuration (1e7 photons per step) is measured beside it in the
+plain FP64 form.
 
 A step is one replicated time step.  Its host part (step physics, upload of
 cells and per-cell source energies, tally copy-out, temperature update) runs
 untimed; the timed part is the device phase with every input resident in HBM:
 source sampling + photon transport (the kernels write the census) + census
-order and totals (bh_driver_transport), bracketed by device synchronisation.  value = histories transported (new +
-census photons) / summed device-phase seconds of the plain FP64 form (FP64
-tracking that replays the reference's RNG streams and arithmetic; persistent
-waves -- the one-work-item-per-history form gives bit-identical events and is
-reported beside it, and so are the same steps with each device phase launched
-as one HIP graph).
+order and totals (bh_driver_transport), bracketed by device synchronisation.
+Histories = photons transported (new + census).
 
-The JSON line also carries the chosen form (FP32 tracking, FP64 energies and
-tallies) with its per-cell tally error against the FP64 form, segments/s, the
-roofline of the transport kernel (algorithmic bytes per launch over its
-HIP-event time against 8 TB/s HBM; measured HBM traffic per launch from the
-committed rocprofv3 counter summary) and the oracle's OpenMP CPU step
-(source + transport + census) timed on this host over the same workload.
+The JSON line also carries segments/s, the same steps launched as one HIP
+graph per step, whole steps with the step physics on the device, the roofline
+of the dominant kernel (the transport kernel: algorithmic bytes per launch
+over its HIP-event time against 8 TB/s HBM; measured HBM traffic per launch
+from the committed rocprofv3 counter summary of the same launches, when that
+summary was made with the same --steps and --warmup) and the oracle's OpenMP
+CPU step (source + transport + census) timed on this host.
 """
 from __future__ import annotations
 
@@ -42,9 +47,6 @@ METRIC = "histories/s & segments/s on 1 MI355X, plain FP64 vs chosen at fixed ta
 HBM_PEAK_GBPS = 8000.0
 DT = 0.001  # the deck's dt_start; t_mult = 1
 N_PHOTONS = 10_000_000  # the deck's photons per step
-# measured HBM bytes per transport launch (rocprofv3 FETCH_SIZE/WRITE_SIZE,
-# calibrated against the copy kernel) for this workload and command
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "transport_persistent_fp64_cube40_traffic.md")
 
 # Algorithmic bytes of one transport launch (per history / segment / cell
 # crossing / tally flush): photon state in + out (FP64 SoA, 97 B each way),
@@ -63,7 +65,8 @@ def _algorithmic_bytes(form, hist, segs, flushes):
 
 
 def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=DECK,
-              n_photons=N_PHOTONS, n_groups=1, fp32_tol=0.0, step_graph=False):
+              n_photons=N_PHOTONS, n_groups=1, fp32_tol=0.0, step_graph=False,
+              census_steps=8):
     """Warm up on one driver, then time `steps` steps on a fresh one.  Photon
     buffers are reserved up front (device memory is set-up, not step work):
     the census grows to ~3.7x the photons per step over five steps.
@@ -71,14 +74,15 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
     first two graph steps, one graph per parity of the photon stores; the
     captures are inside the timed steps); transport_ms then times the whole
     graph."""
-    reserve = n_photons * (1 + min(steps, 8))
+    reserve = n_photons * (1 + min(steps, census_steps))
     kw = dict(precision=precision, algorithm=algorithm, n_groups=n_groups, fp32_tol=fp32_tol,
               step_graph=step_graph)
     if warmup:
         with capi.Driver(deck, t_stop=DT * warmup, **kw) as w:
             w.run()
     out = dict(hist=0, segs=0, flushes=0, dev_s=0.0, step_s=0.0, k_ms=[], launches=[],
-               step1=None, replayed=0, lattice=None, tally=None)
+               step1=None, replayed=0, lattice=None, tally=None, src_ms=[], cen_ms=[],
+               replay_ms=[])
     with capi.Driver(deck, t_stop=DT * steps, reserve_photons=reserve, **kw) as d:
         eng = capi.lib()
         h = eng.bh_driver_engine(d._d)
@@ -104,17 +108,21 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
             # transport_ms: HIP events around the transport kernel on the
             # engine's stream
             out["k_ms"].append(r["transport_ms"])
+            out["src_ms"].append(r["source_ms"])
+            out["cen_ms"].append(r["census_ms"])
+            out["replay_ms"].append(r["replay_ms"])
             out["launches"].append((r["transported"], r["n_segments"], r["n_flushes"]))
     return out
 
 
-def _run_resident(capi, steps, warmup, deck=DECK, n_photons=N_PHOTONS, n_groups=1):
+def _run_resident(capi, steps, warmup, deck=DECK, n_photons=N_PHOTONS, n_groups=1,
+                  census_steps=8):
     """Whole steps with the step physics on the device (driver device_physics):
     wall clock of transport + finish per step, nothing but the step report
     crossing PCIe.  prepare() is untimed: in this mode it uploads the mesh and
     material on the first step (set-up, as the photon store reservation) and
     does nothing after.  Histories per second of whole steps."""
-    reserve = n_photons * (1 + min(steps, 8))
+    reserve = n_photons * (1 + min(steps, census_steps))
     kw = dict(algorithm=capi.ALGO_PERSISTENT, n_groups=n_groups, device_physics=True)
     if warmup:
         with capi.Driver(deck, t_stop=DT * warmup, **kw) as w:
@@ -140,15 +148,23 @@ def _roofline(form, run):
     return bytes_mean, ms_mean, bytes_mean / (ms_mean * 1e-3) / 1e9
 
 
-def _traffic_from_profile():
-    """HBM bytes per launch read from the committed counter summary (the
-    line 'traffic_bytes_per_launch_mean: <bytes>'); None when absent."""
+def _traffic_from_profile(path, steps, warmup):
+    """Mean HBM bytes per timed transport launch from a committed counter
+    summary, when it was made over the same launches: the summary's line
+    'bench_args: --steps K --warmup W' must match and its line
+    'traffic_bytes_per_launch: [...]' hold one value per timed step.  None
+    otherwise."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
-            m = re.search(r"traffic_bytes_per_launch_mean:\s*([0-9.eE+]+)", f.read())
-        return float(m.group(1)) if m else None
+        with open(path) as f:
+            text = f.read()
     except OSError:
         return None
+    m = re.search(r"bench_args:\s*--steps\s+(\d+)\s+--warmup\s+(\d+)", text)
+    t = re.search(r"traffic_bytes_per_launch:\s*\[([^\]]*)\]", text)
+    if not m or not t or (int(m.group(1)), int(m.group(2))) != (steps, warmup):
+        return None
+    vals = [float(x) for x in t.group(1).split(",") if x.strip()]
+    return sum(vals) / len(vals) if len(vals) == steps else None
 
 
 def _tally_error(ref, got):
@@ -165,16 +181,39 @@ def _tally_error(ref, got):
             "zero_cells_differ": int(np.count_nonzero(got[~m]))}
 
 
-def _cpu_baseline(steps):
-    """The oracle's OpenMP step (schedule(guided), per-thread tallies): the
-    same source + transport + census phases over the same workload, on this
-    host's cores."""
+def _host_cores():
+    """Physical cores of this host (unique core/socket pairs of lscpu -p),
+    or the logical CPU count when lscpu is missing."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        pairs = {l.strip() for l in out.splitlines() if l.strip() and not l.startswith("#")}
+        if pairs:
+            return len(pairs)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count() or 1
+
+
+def _cpu_threads():
+    """Every physical core, within the CPU share this process is given
+    (OMP_NUM_THREADS when set: the GPU boxes allot 16 cores per GPU)."""
+    phys = _host_cores()
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(phys, env) if env > 0 else phys), phys
+
+
+def _cpu_baseline(deck_path, n_groups, n_photons, steps, what):
+    """The oracle's OpenMP step (schedule(guided), per-thread tallies; source
+    serial as the reference's): source + transport + census of the first
+    `steps` steps of the deck, timed on this host."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16))
-    deck = oracle.Deck.from_xml(DECK)
-    deck.t_stop = DT * steps
+    threads, phys = _cpu_threads()
+    deck = oracle.Deck.from_xml(deck_path, n_groups=n_groups)
+    deck.n_photons = n_photons
+    deck.t_stop = deck.t_start + DT * steps
     sim = oracle.Sim(deck)
     hist = segs = 0
     secs = t_secs = 0.0
@@ -185,10 +224,11 @@ def _cpu_baseline(steps):
         secs += r["transport_seconds"] + r["source_seconds"]
         t_secs += r["transport_seconds"]
     return {"value": hist / secs, "unit": "histories/s", "cores": threads, "kind": "port",
+            "host_physical_cores": phys,
             "segments_per_s": segs / secs, "transport_only_histories_per_s": hist / t_secs,
-            "sample": f"all {steps} steps of the same 40^3 cube deck at 1e7 photons/step "
-                      f"({hist} histories, {secs:.2f} s of source + transport + census), "
-                      f"oracle C restatement, OpenMP guided, {threads} threads"}
+            "sample": f"{what}: {hist} histories, {secs:.2f} s of source + transport + census; "
+                      f"oracle C restatement, OpenMP guided, {threads} threads "
+                      f"(lscpu: {phys} physical cores on this host)"}
 
 
 HOHLRAUM = os.path.join(ROOT, "branson_amd", "decks", "hohlraum_64.xml")
@@ -205,61 +245,146 @@ def _common(metric, value, steps, warmup, ms, dtype, config, **extra):
     return line
 
 
-def _bench_hohlraum(a, capi):
-    """30-group 64^3 hohlraum, 5e7 photons/step: plain FP64 against FP32
-    tracking (persistent waves) at the stated per-cell tally error."""
-    n_ph = 50_000_000
-    kw = dict(deck=HOHLRAUM, n_photons=n_ph, n_groups=30, keep_step1=True)
-    plain = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT, **kw)
-    ch = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP32_TRACK, capi.ALGO_PERSISTENT,
-                   fp32_tol=a.fp32_tol, **kw)
-    hist = plain["hist"] / plain["dev_s"]
-    chosen = {"form": "fp32 tracking + fp64 energies and tallies, persistent waves",
-              "fp32_tol": a.fp32_tol or 1e-6,
-              "histories_per_s": ch["hist"] / ch["dev_s"],
-              "segments_per_s": ch["segs"] / ch["dev_s"],
-              "ms_per_step": 1e3 * ch["dev_s"] / a.steps,
-              "speedup_vs_plain_fp64": (ch["hist"] / ch["dev_s"]) / hist,
-              "step1_tally_error_vs_fp64": _tally_error(plain["step1"], ch["step1"]),
-              "replayed_histories": ch["replayed"]}
-    resident = _run_resident(capi, a.steps, a.warmup, deck=HOHLRAUM, n_photons=n_ph,
-                             n_groups=30)
-    return _common(METRIC, hist, a.steps, a.warmup, 1e3 * plain["dev_s"] / a.steps, "f64",
-                   {"workload": "hohlraum_64: 64^3 cells of 0.01 cm, 4-cell wall sigma_a=1000 "
-                                "rho=19, fill sigma_a=0.1 sigma_s=0.01 rho=0.001, 0.3 keV "
-                                "source on z=0, 30 groups, 5e7 photons/step",
-                    "form": "plain fp64, persistent waves",
-                    "cell_records": "lattice" if plain["lattice"] else "general",
-                    "tally": plain["tally"],
-                    "histories": plain["hist"], "segments": plain["segs"]},
-                   segments_per_s=plain["segs"] / plain["dev_s"],
-                   whole_step_histories_per_s=plain["hist"] / plain["step_s"],
-                   resident_whole_step_histories_per_s=resident, chosen=chosen)
+HOHLRAUM_WORKLOAD = ("hohlraum_64: 64^3 cells of 0.01 cm, 4-cell wall sigma_a=1000 rho=19, "
+                     "fill sigma_a=0.1 sigma_s=0.01 rho=0.001, 0.3 keV source on z=0, 30 "
+                     "groups, 5e7 photons/step, dt=0.001 sh, seed 4321")
+CUBE_WORKLOAD = ("cube_40_throughput: plain gray 3D cube, 40x40x40 cells of 0.05 cm, vacuum "
+                 "faces, T=1 keV, emission only, sigma_a=5, sigma_s=1, dt=0.001 sh, 1e7 "
+                 "photons/step, seed 1234")
+HOHLRAUM_PHOTONS = 50_000_000
+TALLY_BAR = 1e-6  # per-cell absorbed-energy error the chosen form must meet
+HOHLRAUM_TRAFFIC = os.path.join(ROOT, "profiles",
+                                "transport_persistent_fp64_hohlraum64_traffic.md")
+
+
+def _mean(x):
+    return sum(x) / len(x) if x else 0.0
+
+
+def _kernel_split(run):
+    """Mean device time per step of the source, transport and census phases
+    (HIP events on the engine's stream)."""
+    return {"source_ms": _mean(run["src_ms"]), "transport_ms": _mean(run["k_ms"]),
+            "census_ms": _mean(run["cen_ms"])}
+
+
+def _bench_headline(a, capi, cube=True):
+    """The 30-group hohlraum: plain FP64 against FP32 tracking at fp32_tol;
+    the chosen form is the faster one within TALLY_BAR of FP64 per cell.
+    Beside it: graph and resident whole steps, the roofline of the transport
+    kernel, the cube throughput line and the CPU baseline."""
+    with capi.Engine(0) as e:
+        copy_gbps = e.copy_probe(1 << 30, 20)
+    # the wall absorbs nearly every photon, so the census stays a small
+    # fraction of a step's photons: stores for two steps' worth
+    kw = dict(deck=HOHLRAUM, n_photons=HOHLRAUM_PHOTONS, n_groups=30, census_steps=1)
+    plain = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT,
+                      keep_step1=True, **kw)
+    fp32 = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP32_TRACK, capi.ALGO_PERSISTENT,
+                     keep_step1=True, fp32_tol=a.fp32_tol, **kw)
+    plain_rate = plain["hist"] / plain["dev_s"]
+    fp32_rate = fp32["hist"] / fp32["dev_s"]
+    err = _tally_error(plain["step1"], fp32["step1"])
+    fp32_ok = err["max_rel"] <= TALLY_BAR
+    use_fp32 = fp32_ok and fp32_rate > plain_rate
+    ch, ch_rate = (fp32, fp32_rate) if use_fp32 else (plain, plain_rate)
+    fp32_block = {
+        "form": "fp32 tracking + fp64 energies and tallies, persistent waves",
+        "fp32_tol": a.fp32_tol or 1e-6,
+        "histories_per_s": fp32_rate,
+        "segments_per_s": fp32["segs"] / fp32["dev_s"],
+        "ms_per_step": 1e3 * fp32["dev_s"] / a.steps,
+        "speedup_vs_plain_fp64": fp32_rate / plain_rate,
+        "step1_tally_error_vs_fp64": err,
+        "meets_tally_bar": fp32_ok,
+        "replayed_histories": fp32["replayed"],
+        "replay_fraction": fp32["replayed"] / max(fp32["hist"], 1),
+        "kernel_ms_per_step": dict(_kernel_split(fp32),
+                                   fp32_pass_ms=_mean(fp32["k_ms"]) - _mean(fp32["replay_ms"]),
+                                   fp64_replay_ms=_mean(fp32["replay_ms"])),
+    }
+    graph = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT,
+                      step_graph=True, **kw)
+    resident = _run_resident(capi, a.steps, a.warmup, deck=HOHLRAUM,
+                             n_photons=HOHLRAUM_PHOTONS, n_groups=30, census_steps=1)
+    bytes_launch, kms, achieved = _roofline("fp64", plain)
+    line = _common(
+        METRIC, ch_rate, a.steps, a.warmup, 1e3 * ch["dev_s"] / a.steps, "f64",
+        {"workload": HOHLRAUM_WORKLOAD,
+         "form": ("fp32 tracking + fp64 energies and tallies" if use_fp32 else
+                  "plain fp64 (reference RNG streams and arithmetic)") + ", persistent waves",
+         "chosen_by": f"faster of plain fp64 and fp32 tracking with step-1 max per-cell "
+                      f"abs_E error <= {TALLY_BAR:g} against fp64",
+         "cell_records": "lattice" if ch["lattice"] else "general", "tally": ch["tally"],
+         "histories": ch["hist"], "segments": ch["segs"]},
+        segments_per_s=ch["segs"] / ch["dev_s"],
+        plain_fp64={"histories_per_s": plain_rate,
+                    "segments_per_s": plain["segs"] / plain["dev_s"],
+                    "ms_per_step": 1e3 * plain["dev_s"] / a.steps,
+                    "kernel_ms_per_step": _kernel_split(plain)},
+        fp32_tracking=fp32_block,
+        graph_steps={"histories_per_s": graph["hist"] / graph["dev_s"],
+                     "ms_per_step": 1e3 * graph["dev_s"] / a.steps},
+        whole_step_histories_per_s=plain["hist"] / plain["step_s"],
+        resident_whole_step_histories_per_s=resident,
+        roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBPS,
+                  "traffic": _traffic_from_profile(HOHLRAUM_TRAFFIC, a.steps, a.warmup),
+                  "kernel": "k_transport_persistent<%s, double, %s> (plain fp64, hohlraum)"
+                            % ("HashTally" if plain["tally"] == TALLY_FORMS[4] else "tally",
+                               "true" if plain["lattice"] else "false"),
+                  "kernel_ms_mean": kms, "algorithmic_bytes_per_launch": bytes_launch,
+                  "launches": len(plain["k_ms"]),
+                  "launch_histories_segments_flushes": plain["launches"],
+                  "launch_kernel_ms": plain["k_ms"], "copy_probe_GBps": copy_gbps})
+    if cube:
+        c = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT)
+        cb, cms, cach = _roofline("fp64", c)
+        line["cube_throughput"] = {
+            "workload": CUBE_WORKLOAD, "form": "plain fp64, persistent waves",
+            "histories_per_s": c["hist"] / c["dev_s"], "segments_per_s": c["segs"] / c["dev_s"],
+            "ms_per_step": 1e3 * c["dev_s"] / a.steps, "tally": c["tally"],
+            "histories": c["hist"], "segments": c["segs"],
+            "kernel_ms_per_step": _kernel_split(c),
+            "transport_kernel_roofline": {"achieved_GBps": cach, "frac": cach / HBM_PEAK_GBPS,
+                                          "kernel_ms_mean": cms,
+                                          "algorithmic_bytes_per_launch": cb}}
+    line["cpu_baseline"] = None if a.no_cpu else _cpu_baseline(
+        HOHLRAUM, 30, HOHLRAUM_PHOTONS, 1,
+        "step 1 of the same 30-group hohlraum deck at 5e7 photons/step")
+    return line
 
 
 def _bench_batched(a, capi):
-    """256 independent slab/cube problems in one engine, 3 steps."""
-    import numpy as np
+    """256 independent slab/cube problems in one engine, 3 steps, each step
+    one bh_step_resident_instances call (physics on the device; steps after
+    the first one graph launch each)."""
     from branson_amd import problems
     from branson_amd.batched import BatchedRun
 
     steps = min(a.steps, 3)
-    res = {}
+    res, step1 = {}, {}
     for prec in (capi.PRECISION_FP64, capi.PRECISION_FP32_TRACK):
         b = problems.batched_instances()
         with capi.Engine(0, algorithm=capi.ALGO_PERSISTENT, precision=prec,
                          fp32_tol=a.fp32_tol,
                          reserve_photons=int(b.n_photons.sum()) * 3) as e:
             run = BatchedRun(b, e)
-            reps = [run.step() for _ in range(steps)]
+            reps = []
+            for k in range(steps):
+                reps.append(run.step())
+                if k == 0:
+                    step1[prec] = run.tallies()[0]
         res[prec] = reps
+
     def rate(reps):
         return sum(r["n_transported"] for r in reps) / sum(r["device_s"] for r in reps)
     p64, p32 = res[capi.PRECISION_FP64], res[capi.PRECISION_FP32_TRACK]
     chosen = {"form": "fp32 tracking + fp64 energies and tallies, persistent waves",
               "fp32_tol": a.fp32_tol or 1e-6, "histories_per_s": rate(p32),
               "speedup_vs_plain_fp64": rate(p32) / rate(p64),
-              "step1_tally_error_vs_fp64": _tally_error(p64[0]["abs_E"], p32[0]["abs_E"]),
+              "step1_tally_error_vs_fp64": _tally_error(step1[capi.PRECISION_FP64],
+                                                        step1[capi.PRECISION_FP32_TRACK]),
               "replayed_histories": sum(r["n_replayed"] for r in p32)}
     return _common(METRIC, rate(p64), steps, 0,
                    1e3 * sum(r["device_s"] for r in p64) / steps, "f64",
@@ -268,9 +393,11 @@ def _bench_batched(a, capi):
                                 "lognormal (median 10, sigma_log 1), sigma_s U[0,2], T0 "
                                 "U[0.01,1] keV, dt 0.01 sh, seeds 0..255, one launch per "
                                 "phase for all instances",
-                    "form": "plain fp64, persistent waves",
+                    "form": "plain fp64, persistent waves, whole steps (physics on the "
+                            "device, steps after the first one graph launch)",
                     "histories": sum(r["n_transported"] for r in p64),
                     "segments": sum(r["n_segments"] for r in p64)},
+                   max_instance_balance=max(r["max_instance_balance"] for r in p64),
                    chosen=chosen)
 
 
@@ -309,20 +436,53 @@ def _bench_adjoint(a, capi):
                               "fleck": float(np.linalg.norm(df))})
 
 
+def _bench_cube(a, capi):
+    """The 40^3 cube throughput configuration alone, plain FP64, with the
+    one-work-item-per-history form and FP32 tracking beside it."""
+    with capi.Engine(0) as e:
+        copy_gbps = e.copy_probe(1 << 30, 20)
+    plain = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT,
+                      keep_step1=True)
+    value = plain["hist"] / plain["dev_s"]
+    bytes_launch, kms, achieved = _roofline("fp64", plain)
+    hi = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_HISTORY)
+    ch = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP32_TRACK, capi.ALGO_PERSISTENT,
+                   keep_step1=True, fp32_tol=a.fp32_tol)
+    return _common(
+        METRIC, value, a.steps, a.warmup, 1e3 * plain["dev_s"] / a.steps, "f64",
+        {"workload": CUBE_WORKLOAD, "form": "plain fp64, persistent waves",
+         "cell_records": "lattice" if plain["lattice"] else "general", "tally": plain["tally"],
+         "histories": plain["hist"], "segments": plain["segs"]},
+        segments_per_s=plain["segs"] / plain["dev_s"],
+        kernel_ms_per_step=_kernel_split(plain),
+        history_form={"form": "plain fp64, one work-item per history (bit-identical events)",
+                      "histories_per_s": hi["hist"] / hi["dev_s"],
+                      "transport_kernel_ms_mean": _roofline("fp64", hi)[1]},
+        fp32_tracking={"histories_per_s": ch["hist"] / ch["dev_s"],
+                       "step1_tally_error_vs_fp64": _tally_error(plain["step1"], ch["step1"]),
+                       "replay_fraction": ch["replayed"] / max(ch["hist"], 1)},
+        roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                  "kernel": "k_transport_persistent (plain fp64, cube)",
+                  "kernel_ms_mean": kms, "algorithmic_bytes_per_launch": bytes_launch,
+                  "launches": len(plain["k_ms"]), "copy_probe_GBps": copy_gbps},
+        cpu_baseline=None if a.no_cpu else _cpu_baseline(
+            DECK, 1, N_PHOTONS, 2, "steps 1-2 of the same 40^3 cube deck at 1e7 photons/step"))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-chosen", action="store_true",
-                    help="skip the chosen form and the history-form comparison")
-    ap.add_argument("--workload", default="cube",
-                    choices=["cube", "hohlraum", "batched", "adjoint"],
-                    help="cube (default): the throughput line; the others measure the "
-                         "30-group hohlraum, the 256-problem batch and the adjoint")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "cube", "batched", "adjoint"],
+                    help="headline (default): the 30-group hohlraum, plain FP64 against "
+                         "FP32 tracking, with the cube throughput beside it; cube: the cube "
+                         "alone; batched: the 256-problem batch; adjoint: the adjoint")
     ap.add_argument("--fp32-tol", type=float, default=0.0,
-                    help="FP32 tracking: per-deposit relative error bound (0 = default)")
+                    help="FP32 tracking: per-deposit relative error bound (0 = default 1e-6)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: every run "
@@ -333,99 +493,10 @@ def main(argv=None):
         sys.exit("--steps must be >= 1")
 
     from branson_amd import capi
-    import numpy as np
 
-    if a.workload != "cube":
-        fn = {"hohlraum": _bench_hohlraum, "batched": _bench_batched,
-              "adjoint": _bench_adjoint}[a.workload]
-        print(json.dumps(fn(a, capi)), flush=True)
-        return
-
-    # streaming-copy probe: the HBM rate this device reaches with a plain
-    # copy kernel (k_copy_probe, 1 GiB each way), also the calibration kernel
-    # for the FETCH_SIZE / WRITE_SIZE counters of the profiles
-    with capi.Engine(0) as e:
-        copy_gbps = e.copy_probe(1 << 30, 20)
-
-    plain = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT,
-                      keep_step1=True)
-    value = plain["hist"] / plain["dev_s"]
-    bytes_launch, kms, achieved = _roofline("fp64", plain)
-    # the same steps with each device phase as one HIP graph (the two
-    # captures fall inside the timed steps)
-    graph = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT,
-                      step_graph=True)
-
-    history = chosen = None
-    if not a.no_chosen:
-        hi = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_HISTORY)
-        history = {"form": "plain fp64, one work-item per history (bit-identical events)",
-                   "histories_per_s": hi["hist"] / hi["dev_s"],
-                   "segments_per_s": hi["segs"] / hi["dev_s"],
-                   "ms_per_step": 1e3 * hi["dev_s"] / a.steps,
-                   "transport_kernel_ms_mean": _roofline("fp64", hi)[1]}
-        ch = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP32_TRACK,
-                       capi.ALGO_PERSISTENT, keep_step1=True, fp32_tol=a.fp32_tol)
-        cb, cms, cach = _roofline("fp32", ch)
-        chosen = {
-            "form": "fp32 tracking + fp64 energies and tallies, persistent waves",
-            "histories_per_s": ch["hist"] / ch["dev_s"],
-            "segments_per_s": ch["segs"] / ch["dev_s"],
-            "ms_per_step": 1e3 * ch["dev_s"] / a.steps,
-            "step1_tally_error_vs_fp64": _tally_error(plain["step1"], ch["step1"]),
-            "replayed_histories": ch["replayed"],
-            "transport_kernel_ms_mean": cms,
-            "transport_kernel_GBps_algorithmic": cach,
-        }
-
-    cpu = None if a.no_cpu else _cpu_baseline(a.steps)
-
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "histories/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": 1e3 * plain["dev_s"] / a.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "cube_40_throughput: plain gray 3D cube, 40x40x40 cells of "
-                               "0.05 cm, vacuum faces, T=1 keV, emission only, sigma_a=5, "
-                               "sigma_s=1, dt=0.001 sh, 1e7 photons/step, seed 1234",
-                   "form": "plain fp64 (reference RNG streams and arithmetic), "
-                           "persistent waves",
-                   "cell_records": "lattice" if plain["lattice"] else "general",
-                   "tally": plain["tally"],
-                   "histories": plain["hist"], "segments": plain["segs"]},
-        "segments_per_s": plain["segs"] / plain["dev_s"],
-        # photon counts follow temperatures that carry the tally atomics'
-        # round-off, so the graph run may differ by a photon in a rare cell
-        "graph_steps": {"histories_per_s": graph["hist"] / graph["dev_s"],
-                        "ms_per_step": 1e3 * graph["dev_s"] / a.steps,
-                        "histories_segments": [graph["hist"], graph["segs"]]},
-        # whole step: host physics, uploads of cells and source energies over
-        # PCIe, device phase, tally copy-out and temperature update
-        "whole_step_histories_per_s": plain["hist"] / plain["step_s"],
-        # whole step with the step physics on the device (no per-step uploads)
-        "resident_whole_step_histories_per_s": _run_resident(capi, a.steps, a.warmup),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": _traffic_from_profile(),
-                     "kernel": "k_transport_persistent<GlobalTally<false>, double, %s>"
-                               % ("true" if plain["lattice"] else "false"),
-                     "kernel_ms_mean": kms, "algorithmic_bytes_per_launch": bytes_launch,
-                     "launches": len(plain["k_ms"]),
-                     "launch_histories_segments_flushes": plain["launches"],
-                     "launch_kernel_ms": plain["k_ms"], "copy_probe_GBps": copy_gbps},
-        "cpu_baseline": cpu,
-        "history_form": history,
-        "chosen": chosen,
-    }
-    print(json.dumps(line), flush=True)
+    fn = {"headline": _bench_headline, "cube": _bench_cube, "batched": _bench_batched,
+          "adjoint": _bench_adjoint}[a.workload]
+    print(json.dumps(fn(a, capi)), flush=True)
 
 
 if __name__ == "__main__":

@@ -36,7 +36,7 @@ EXPORTS = [
     "bh_abi_version", "bh_create", "bh_destroy", "bh_last_error", "bh_sync", "bh_set_mesh",
     "bh_set_source_energy", "bh_source", "bh_source_from_energy", "bh_set_instances",
     "bh_source_instances", "bh_transport", "bh_set_material", "bh_step_resident",
-    "bh_get_temperature", "bh_get_physics", "bh_comb_census",
+    "bh_get_temperature", "bh_get_physics", "bh_comb_census", "bh_step_resident_instances",
     "bh_step_graph", "bh_get_tallies", "bh_census_size", "bh_census_export",
     "bh_photons_export",
     "bh_census_import",
@@ -78,6 +78,12 @@ class StepStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class InstanceReport(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("total_E", "emission_E", "source_E", "pre_census_E",
+                                          "post_census_E", "absorbed_E", "exit_E",
+                                          "pre_mat_E", "post_mat_E")] + [("n_new", C.c_uint64)]
+
+
 class Material(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("opA", "opB", "opC", "opS", "cv", "rho", "T_e", "T_r",
                                           "T_s", "source_face")]
@@ -87,7 +93,7 @@ class ResidentReport(C.Structure):
     _fields_ = [("total_E", C.c_double), ("emission_E", C.c_double), ("source_E", C.c_double),
                 ("pre_census_E", C.c_double), ("pre_mat_E", C.c_double),
                 ("post_mat_E", C.c_double), ("absorbed_E", C.c_double), ("stats", StepStats),
-                ("n_new", C.c_uint64), ("etot_serial", C.c_int32), ("pad", C.c_int32)]
+                ("n_new", C.c_uint64), ("etot_serial", C.c_int32), ("graph", C.c_int32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stats"}
@@ -144,6 +150,7 @@ def lib() -> C.CDLL:
     L.bh_copy_probe.argtypes = [P, U64, I, P]
     L.bh_set_material.argtypes = [P, P, D, U32, U32]
     L.bh_step_resident.argtypes = [P, U64, U32, U32, U32, D, D, P]
+    L.bh_step_resident_instances.argtypes = [P, P, U32, U32, D, D, P, P]
     L.bh_get_temperature.argtypes = [P, P, P]
     L.bh_get_physics.argtypes = [P, P, P, P, P, P]
     L.bh_comb_census.argtypes = [P, C.c_int64, U32, U64, U64, P]
@@ -370,6 +377,35 @@ class Engine:
                                            C.byref(rep)))
         self._n_new = 0
         return rep.as_dict()
+
+    def physics(self):
+        """bh_get_physics: the opacities, Fleck factors and source energies
+        of the last resident step (or of the last bh_set_mesh / energies)."""
+        n, G = self.n_cells, self.n_groups
+        sa, ss = np.zeros((n, G)), np.zeros((n, G))
+        f, em, src = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._check(lib().bh_get_physics(self._h, _p(sa), _p(ss), _p(f), _p(em), _p(src)))
+        return dict(sigma_a=sa, sigma_s=ss, fleck=f, emission_E=em, source_E=src)
+
+    def step_resident_instances(self, n_user, step, rank, dt, next_dt):
+        """bh_step_resident_instances: one step of every instance, physics
+        included, on the device.  Returns (total report, per-instance
+        structured array)."""
+        k = len(self._inst_keep[1]) if getattr(self, "_inst_keep", None) else 0
+        nu = np.ascontiguousarray(n_user, dtype=np.uint64)
+        if len(nu) != k:
+            raise ValueError("one photon count per instance")
+        rep = ResidentReport()
+        per = (InstanceReport * k)()
+        self._check(lib().bh_step_resident_instances(self._h, _p(nu), step, rank, dt, next_dt,
+                                                     C.byref(rep), per))
+        self._n_new = 0
+        names = [n for n, _ in InstanceReport._fields_]
+        arr = np.zeros(k, dtype=[(n, np.uint64 if n == "n_new" else np.float64)
+                                 for n in names])
+        for n in names:
+            arr[n] = [getattr(r, n) for r in per]
+        return rep.as_dict(), arr
 
     def temperature(self):
         T_e, T_r = np.zeros(self.n_cells), np.zeros(self.n_cells)

@@ -185,6 +185,12 @@ struct bh_handle {
   DArr<uint32_t> cell_inst;
   DArr<uint64_t> inst_ctr_hi;
   DArr<SourceInst> inst_src;            // [2 * n_inst]: census, then new photons
+  DArr<uint32_t> inst_first_dev;        // n_inst + 1 cell bounds on the device
+  DArr<InstReport> inst_rep;            // resident steps: per-instance totals
+  DArr<double> inst_exit;               // resident steps: exit energy per instance
+  SourceInst* h_rec = nullptr;          // pinned: the step's source records
+  InstReport* h_inst_rep = nullptr;     // pinned: per-instance totals read back
+  size_t h_inst_cap = 0;                // instances the two pinned buffers hold
   DArr<uint64_t> rep_idx;    // FP32 form: histories handed to the FP64 replay
   DArr<uint32_t> rep_first;
   DArr<double> rep_E;
@@ -202,6 +208,7 @@ struct bh_handle {
     std::vector<uintptr_t> key;  // the buffers and sizes it was captured with
   };
   StepGraph graphs[2];
+  StepGraph resident_graphs[2];    // bh_step_resident(_instances) steps after the first
   StepParams* d_params = nullptr;  // what the graph's kernels read
   StepParams* h_params = nullptr;  // pinned; the graph's first node copies it in
   double* h_tally = nullptr;       // pinned: abs_E then track_E
@@ -210,6 +217,7 @@ struct bh_handle {
 
   hipEvent_t ev[6] = {};
   bool replay_timed = false;  // ev[3] marks the start of the last FP64 replay
+  bool inst_exit_on = false;  // bh_transport adds exit energy per instance
   double last_source_ms = 0.0;
   int n_cu = 256;
   size_t lds_budget = 48 * 1024;  // dynamic LDS a tally may use per block
@@ -333,8 +341,10 @@ int scan_u64(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out, int sl
              const uint64_t* n_ptr = nullptr, bool nonzero = false,
              double* E_total = nullptr) {
   const uint64_t n_tiles = std::max<uint64_t>(div_up(n, kScanTile), 1);
-  // the single-block middle pass loops over the tile sums; keep it short
-  if (n_tiles > 65536)
+  // the single-block middle pass loops over the tile sums (of the count read
+  // on the device where there is one: n may be a capacity); 2^20 tiles hold
+  // every 32-bit photon index
+  if (n_tiles > ((uint64_t)1 << 20))
     return fail(h, BH_ERR_ARG, "scan of %llu elements exceeds the supported size",
                 (unsigned long long)n);
   TRY(ensure(h, h->tiles, n_tiles));
@@ -866,6 +876,15 @@ void bh_destroy(bh_handle* h) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
   }
+  for (auto& g : h->resident_graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  release(h->inst_first_dev);
+  release(h->inst_rep);
+  release(h->inst_exit);
+  if (h->h_rec) (void)hipHostFree(h->h_rec);
+  if (h->h_inst_rep) (void)hipHostFree(h->h_inst_rep);
   if (h->d_params) (void)hipFree(h->d_params);
   if (h->h_params) (void)hipHostFree(h->h_params);
   if (h->h_tally) (void)hipHostFree(h->h_tally);
@@ -1104,6 +1123,24 @@ int bh_set_instances(bh_handle* h, uint32_t n_instances, const uint32_t* first_c
   TRY(ensure(h, h->cell_inst, h->n_cells));
   TRY(ensure(h, h->inst_ctr_hi, n_instances));
   TRY(ensure(h, h->inst_src, 2 * (size_t)n_instances));
+  TRY(ensure(h, h->inst_first_dev, (size_t)n_instances + 1));
+  TRY(ensure(h, h->inst_rep, n_instances));
+  TRY(ensure(h, h->inst_exit, n_instances));
+  if (h->h_inst_cap < n_instances) {
+    if (h->h_rec) HIPCHK(h, hipHostFree(h->h_rec));
+    if (h->h_inst_rep) HIPCHK(h, hipHostFree(h->h_inst_rep));
+    h->h_rec = nullptr;
+    h->h_inst_rep = nullptr;
+    h->h_inst_cap = 0;
+    HIPCHK(h, hipHostMalloc(&h->h_rec, 2 * (size_t)n_instances * sizeof(SourceInst),
+                            hipHostMallocDefault));
+    HIPCHK(h, hipHostMalloc(&h->h_inst_rep, (size_t)n_instances * sizeof(InstReport),
+                            hipHostMallocDefault));
+    h->h_inst_cap = n_instances;
+  }
+  HIPCHK(h, hipMemcpyAsync(h->inst_first_dev.p, first_cell,
+                           ((size_t)n_instances + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           h->stream));
   HIPCHK(h, hipMemcpyAsync(h->cell_inst.p, h->cell_inst_host.data(),
                            h->n_cells * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->inst_ctr_hi.p, h->inst_hi.data(), n_instances * sizeof(uint64_t),
@@ -1159,6 +1196,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   // restored from the transport indices afterwards
   CensusOut co;
   TRY(census_prepare(h, n, next_dt, co));
+  co.inst_exit = (h->inst_exit_on && h->n_inst) ? h->inst_exit.p : nullptr;
   h->replay_timed = false;
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
   TRY(launch_transport(h, h->W.soa, n, nullptr, co, handle_sinks(h, h->ctr_hi)));
@@ -1974,7 +2012,8 @@ int bh_set_material(bh_handle* h, const bh_material* m, double replicated_factor
                     uint32_t n_ranks, uint32_t rank) {
   if (!h || !h->stream || !m) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_material before bh_set_mesh");
-  if (h->n_inst) return fail(h, BH_ERR_STATE, "resident step physics runs one problem");
+  if (h->n_inst && (n_ranks != 1 || replicated_factor != 1.0))
+    return fail(h, BH_ERR_ARG, "instances run as one rank (replicated_factor 1)");
   if (n_ranks == 0 || rank >= n_ranks || !(replicated_factor > 0.0))
     return fail(h, BH_ERR_ARG, "need n_ranks >= 1, rank < n_ranks, replicated_factor > 0");
   if (n_ranks > kMaxRanks) return fail(h, BH_ERR_ARG, "at most %u ranks", kMaxRanks);
@@ -1999,11 +2038,15 @@ int bh_set_material(bh_handle* h, const bh_material* m, double replicated_factor
   TRY(ensure(h, h->face, n));
   if (!h->d_phys) HIPCHK(h, hipMalloc(&h->d_phys, sizeof(PhysTotals)));
   if (!h->d_inst2) HIPCHK(h, hipMalloc(&h->d_inst2, 2 * sizeof(SourceInst)));
+  if (!h->d_params) HIPCHK(h, hipMalloc(&h->d_params, sizeof(StepParams)));
+  if (!h->h_params)
+    HIPCHK(h, hipHostMalloc(&h->h_params, sizeof(StepParams), hipHostMallocDefault));
   for (int k = 0; k < 9; ++k)
     HIPCHK(h, hipMemcpyAsync(h->mat.p + (size_t)k * n, cols[k], n * sizeof(double),
                              hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->face.p, m->source_face, n, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemsetAsync(h->tr_diag.p, 0, n * sizeof(double), h->stream));
+  k_fill_f64<<<grid_for(n, 256), 256, 0, h->stream>>>(h->tr_diag.p, n, 0.0);
+  HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->mat_rf = replicated_factor;
   h->mat_ranks = n_ranks;
@@ -2012,16 +2055,448 @@ int bh_set_material(bh_handle* h, const bh_material* m, double replicated_factor
   return BH_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Blocks of the deterministic cell sums of a resident step.
+uint32_t resident_blocks(const bh_handle* h) {
+  return std::max<uint32_t>(1u, std::min<uint32_t>(kSumMaxBlocks, div_up(h->n_cells, 4096)));
+}
+
+// Mesh::calculate_photon_energy on the device, with no host round trip:
+// per-cell opacities, Fleck factors and source energies, the step totals,
+// and E_tot into the source records `rec` (one problem: [census, new];
+// instances: [K census | K new]).  One problem and one rank: E_tot summed in
+// parallel with a bound, the reference's serial cell-order sum run on the
+// device only when a photon count could depend on the difference; several
+// ranks: the serial sums, every rank's reassigned total and this rank's
+// share; instances: each instance's own totals (k_inst_sums).  P: a step
+// captured as a graph (dt read on the device).
+int resident_physics(bh_handle* h, bool first, uint64_t n_user, double dt, const StepParams* P,
+                     SourceInst* rec, uint32_t nb) {
+  const uint32_t n = h->n_cells;
+  const MatDev md = mat_dev(h);
+  k_phys_cells<<<grid_for(n, 256), 256, 0, h->stream>>>(
+      h->geo.p, h->sigma.p, h->lattice ? h->rec.p : nullptr, h->G, n, md, dt, h->mat_rf,
+      first ? 1 : 0, h->em.p, h->src.p, h->cen.p, h->mat_e.p, P);
+  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_phys),
+                                      sizeof(PhysTotals) / 8);
+  HIPCHK(h, hipGetLastError());
+  if (h->n_inst) {
+    const uint32_t K = h->n_inst;
+    k_fill_f64<<<grid_for(K, 256), 256, 0, h->stream>>>(h->inst_exit.p, K, 0.0);
+    k_inst_sums<<<K, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p,
+                                            h->inst_first_dev.p, K, h->inst_rep.p, rec);
+    HIPCHK(h, hipGetLastError());
+    return BH_OK;
+  }
+  if (h->mat_ranks == 1) {
+    // E_tot in parallel with a bound on the serial loop's value, a check that
+    // no photon count depends on where in the bound it lies, and the serial
+    // sums only when one does
+    k_phys_partials<<<nb, kSumNT, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
+                                                  h->sum_part.p);
+    k_final_sums<7><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->tot_emission);
+    k_phys_etot<<<1, 1, 0, h->stream>>>(n, h->d_phys, rec);
+    k_count_check<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, n,
+                                                           n_user, first ? 1 : 0, h->d_phys);
+    k_phys_totals<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n, 1, 0,
+                                              h->d_phys, rec, 1);
+  } else {  // the reference's serial sums, then this rank's share of the cells
+    k_phys_totals<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
+                                              h->mat_ranks, 0, h->d_phys, rec, 0);
+    k_phys_rank_sums<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, n, n_user,
+                                                 h->mat_ranks, h->mat_rf, first ? 1 : 0,
+                                                 h->d_phys, rec);
+    k_phys_reassign<<<grid_for(n, 256), 256, 0, h->stream>>>(
+        h->em.p, h->src.p, h->cen.p, n, n_user, h->mat_ranks, h->mat_rank, h->mat_rf,
+        first ? 1 : 0, h->d_phys);
+    k_phys_totals<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
+                                              h->mat_ranks, 1, h->d_phys, rec, 0);
+  }
+  HIPCHK(h, hipGetLastError());
+  return BH_OK;
+}
+
+// Mesh::update_temperature on the device and the step's report sums; with
+// instances, each instance's census energy (photons C[0, *n_census)) and
+// totals.
+int resident_finish(bh_handle* h, double dt, const StepParams* P, uint32_t nb,
+                    const PhotonSoA& census, uint64_t census_cap,
+                    const unsigned long long* n_census) {
+  const uint32_t n = h->n_cells;
+  k_update_temperature<<<grid_for(n, 256), 256, 0, h->stream>>>(
+      h->geo.p, mat_dev(h), h->abs_E.p, h->track_E.p, h->em.p, dt, n, h->tr_diag.p,
+      h->mat_e.p, P);
+  k_report_partials<<<nb, kSumNT, 0, h->stream>>>(h->abs_E.p, h->mat_e.p, n, h->sum_part.p);
+  k_final_sums<2><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->absorbed);
+  if (h->n_inst) {
+    const uint32_t K = h->n_inst;
+    k_inst_census<<<std::min<unsigned>(grid_for(std::max<uint64_t>(census_cap, 1), 256), 4096),
+                    256, 0, h->stream>>>(census.cell, census.E, census_cap, n_census, nullptr,
+                                         h->cell_inst.p, h->inst_rep.p, 1);
+    k_inst_report<<<K, kWave, 0, h->stream>>>(h->abs_E.p, h->mat_e.p, h->counts.p,
+                                              h->inst_first_dev.p, h->inst_exit.p, K,
+                                              h->inst_rep.p);
+  }
+  HIPCHK(h, hipGetLastError());
+  return BH_OK;
+}
+
+// The per-step source records: census then new photons, per instance.
+void resident_records(const bh_handle* h, const uint64_t* n_user, uint32_t step, uint32_t rank,
+                      SourceInst* out) {
+  const uint32_t K = h->n_inst ? h->n_inst : 1u;
+  const uint64_t hi1 = h->ctr_hi;
+  for (uint32_t k = 0; k < K; ++k) {
+    const uint64_t nu = n_user[k];
+    const uint64_t rank_off = nu * (uint64_t)rank;
+    const uint64_t hi = h->n_inst ? h->inst_hi[k] : hi1;
+    const uint32_t first = h->n_inst ? h->inst_first[k] : 0u;
+    out[k] = SourceInst{nu, 0.0, rank_off, hi, first, 0};
+    out[K + k] = SourceInst{nu, 0.0, 10000000000000ull * (uint64_t)step + rank_off, hi, first, 0};
+  }
+}
+
+void fill_report(const bh_handle* h, const PhysTotals& pt, const bh_step_stats& st,
+                 uint64_t n_transported, uint64_t n_new, bool first, bh_resident_report* rep) {
+  if (!rep) return;
+  std::memset(rep, 0, sizeof *rep);
+  rep->total_E = pt.global_E;
+  rep->emission_E = pt.tot_emission;
+  rep->source_E = pt.tot_source;
+  rep->pre_census_E = first ? pt.pre_census : 0.0;
+  rep->pre_mat_E = pt.pre_mat;
+  rep->post_mat_E = pt.post_mat;
+  rep->absorbed_E = pt.absorbed;
+  rep->stats = st;
+  rep->stats.n_transported = n_transported;
+  rep->n_new = n_new;
+  rep->etot_serial = (h->mat_ranks > 1 || pt.ambiguous) ? 1 : 0;
+}
+
+// Instances: the totals over instances and each instance's own.
+void fill_instance_reports(const bh_handle* h, const InstReport* r, bh_resident_report* tot,
+                           bh_instance_report* per) {
+  const uint32_t K = h->n_inst;
+  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t n_new = 0;
+  for (uint32_t k = 0; k < K; ++k) {
+    if (per) std::memcpy(&per[k], &r[k], sizeof(bh_instance_report));
+    s[0] += r[k].total_E;
+    s[1] += r[k].emission_E;
+    s[2] += r[k].source_E;
+    s[3] += r[k].pre_census_E;
+    s[5] += r[k].absorbed_E;
+    s[7] += r[k].pre_mat_E;
+    s[8] += r[k].post_mat_E;
+    n_new += r[k].n_new;
+  }
+  if (tot) {
+    tot->total_E = s[0];
+    tot->emission_E = s[1];
+    tot->source_E = s[2];
+    tot->pre_census_E = s[3];
+    tot->absorbed_E = s[5];
+    tot->pre_mat_E = s[7];
+    tot->post_mat_E = s[8];
+    tot->n_new = n_new;
+    tot->etot_serial = 0;
+  }
+}
+static_assert(sizeof(InstReport) == sizeof(bh_instance_report),
+              "InstReport mirrors bh_instance_report");
+
+// One resident step, launch by launch (the first step, which makes the
+// initial census, and the history algorithm): host synchronisations only
+// where the photon stores may have to grow.
+int resident_step_eager(bh_handle* h, const uint64_t* n_user, uint32_t step, uint32_t rank,
+                        double dt, double next_dt, bh_resident_report* rep,
+                        bh_instance_report* irep) {
+  const bool first = step == 1;
+  const uint32_t K = h->n_inst;
+  const uint32_t nb = resident_blocks(h);
+  TRY(ensure(h, h->sum_part, (size_t)8 * nb));
+  SourceInst* rec = nullptr;
+  if (K) {
+    resident_records(h, n_user, step, rank, h->h_rec);
+    HIPCHK(h, hipMemcpyAsync(h->inst_src.p, h->h_rec, 2 * (size_t)K * sizeof(SourceInst),
+                             hipMemcpyHostToDevice, h->stream));
+    rec = h->inst_src.p;
+  } else {
+    SourceInst* host2 = reinterpret_cast<SourceInst*>(static_cast<char*>(h->h_pinned) + 1024);
+    resident_records(h, n_user, step, rank, host2);
+    HIPCHK(h, hipMemcpyAsync(h->d_inst2, host2, 2 * sizeof(SourceInst), hipMemcpyHostToDevice,
+                             h->stream));
+    rec = h->d_inst2;
+  }
+  TRY(resident_physics(h, first, n_user[0], dt, nullptr, rec, nb));
+  h->has_energy = true;
+  h->has_census_E = first;
+  uint64_t n_new = 0;
+  const SourceInst unused{};
+  TRY(run_source(h, unused, unused, rec, rec + (K ? K : 1), dt, &n_new,
+                 (first && !K) ? h->sum_part.p : nullptr, nb));
+  if (first && !K) {
+    k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->pre_census);
+    HIPCHK(h, hipGetLastError());
+  }
+  char* hp = static_cast<char*>(h->h_pinned);
+  PhysTotals pt;
+  HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::memcpy(&pt, hp + 2048, sizeof pt);
+  if (!K) {
+    if (!(pt.global_E > 0.0)) return fail(h, BH_ERR_STATE, "no photon energy this step");
+    h->src_neff_frac = pt.e_sq > 0.0 ? (pt.e_sum * pt.e_sum / pt.e_sq) / (double)h->n_cells : 1.0;
+    if (pt.ambiguous || h->mat_ranks > 1) ++h->serial_etot_steps;
+  } else if (h->n_cen) {  // each instance's census photons before transport
+    k_inst_census<<<std::min<unsigned>(grid_for(h->n_cen, 256), 4096), 256, 0, h->stream>>>(
+        h->W.soa.cell, h->W.soa.E, h->n_cen, nullptr, nullptr, h->cell_inst.p, h->inst_rep.p,
+        0);
+    HIPCHK(h, hipGetLastError());
+  }
+  const uint64_t n_transported = h->n_cen + n_new;
+  bh_step_stats st;
+  h->inst_exit_on = K > 0;
+  const int rc = bh_transport(h, next_dt, nullptr, nullptr, &st);
+  h->inst_exit_on = false;
+  TRY(rc);
+  // after the transport the new census is W[0, n_cen)
+  TRY(resident_finish(h, dt, nullptr, nb, h->W.soa, h->n_cen, nullptr));
+  HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost, h->stream));
+  if (K)
+    HIPCHK(h, hipMemcpyAsync(h->h_inst_rep, h->inst_rep.p, (size_t)K * sizeof(InstReport),
+                             hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::memcpy(&pt, hp + 2048, sizeof pt);
+  fill_report(h, pt, st, n_transported, n_new, first, rep);
+  if (K) fill_instance_reports(h, h->h_inst_rep, rep, irep);
+  return BH_OK;
+}
+
+// Everything a captured resident step bakes in.
+std::vector<uintptr_t> resident_graph_key(const bh_handle* h, uint64_t n_user_total) {
+  std::vector<uintptr_t> key = step_graph_key(h);
+  auto u = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+  const uintptr_t more[] = {u(h->mat.p),      u(h->mat_e.p),       u(h->tr_diag.p),
+                            u(h->d_phys),     u(h->d_inst2),       u(h->cen.p),
+                            u(h->inst_src.p), u(h->inst_rep.p),    u(h->inst_exit.p),
+                            u(h->inst_first_dev.p), u(h->cell_inst.p), u(h->h_rec),
+                            u(h->h_inst_rep), (uintptr_t)h->n_inst, (uintptr_t)h->mat_ranks,
+                            (uintptr_t)h->mat_rank, (uintptr_t)n_user_total};
+  key.insert(key.end(), std::begin(more), std::end(more));
+  uint64_t rf_bits;
+  std::memcpy(&rf_bits, &h->mat_rf, sizeof rf_bits);
+  key.push_back((uintptr_t)rf_bits);
+  return key;
+}
+
+// Records a resident step after the first: the step's values come in from
+// pinned memory, every count stays on the device, and one copy at the end
+// takes the report out.  cap bounds the photons of the step.
+int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0) {
+  const uint32_t nc = h->n_cells, K = h->n_inst;
+  const uint32_t nb = resident_blocks(h);
+  const DevCells cells = dev_cells(h);
+  StepParams* P = h->d_params;
+  HIPCHK(h, hipMemcpyAsync(P, h->h_params, sizeof(StepParams), hipMemcpyHostToDevice,
+                           h->stream));
+  SourceInst* rec = K ? h->inst_src.p : h->d_inst2;
+  const size_t n_rec = 2 * (size_t)(K ? K : 1);
+  HIPCHK(h, hipMemcpyAsync(rec, h->h_rec, n_rec * sizeof(SourceInst), hipMemcpyHostToDevice,
+                           h->stream));
+  TRY(resident_physics(h, false, n_user0, 0.0, P, rec, nb));
+  // make_photons: counts, offsets, the step's counts into P, then the photons
+  const SourceInst unused{};
+  const uint32_t* cinst = K ? h->cell_inst.p : nullptr;
+  SourceInst* inew = rec + (K ? K : 1);
+  k_source_counts<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, nc, unused, inew,
+                                                             cinst, 0, h->counts.p);
+  TRY(scan_u64(h, h->counts.p, nc, h->off.p, 1));
+  k_step_counts<<<1, 1, 0, h->stream>>>(P, h->d_total + 1);
+  k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
+      cells, h->em.p, h->src.p, h->face.p, h->off.p, 0, unused, inew, 0.0, 0, h->W.soa, 0, P);
+  HIPCHK(h, hipGetLastError());
+  if (K) {  // each instance's census photons before transport
+    k_inst_census<<<std::min<unsigned>(grid_for(std::max<uint64_t>(cap, 1), 256), 4096), 256, 0,
+                    h->stream>>>(h->W.soa.cell, h->W.soa.E, cap, nullptr, &P->n_cen,
+                                 h->cell_inst.p, h->inst_rep.p, 0);
+    HIPCHK(h, hipGetLastError());
+  }
+  TRY(zero_tallies_and_counters(h));
+  CensusOut co;
+  TRY(census_prepare(h, cap, 0.0, co));
+  co.inst_exit = K ? h->inst_exit.p : nullptr;
+  TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi)));
+  TRY(census_finish(h, cap, &P->n));
+  TRY(resident_finish(h, 0.0, P, nb, h->C.soa, cap, &h->d_cacc->n_census));
+  char* hp = static_cast<char*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(hp + 256, h->d_ctrs, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof(PhysTotals), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(hp + 3072, P, sizeof(StepParams), hipMemcpyDeviceToHost,
+                           h->stream));
+  if (K)
+    HIPCHK(h, hipMemcpyAsync(h->h_inst_rep, h->inst_rep.p, (size_t)K * sizeof(InstReport),
+                             hipMemcpyDeviceToHost, h->stream));
+  return BH_OK;
+}
+
+// A resident step after the first as one HIP graph launch: no host round
+// trip between its phases.  The photon stores are sized beforehand for the
+// most photons the step can make (the census plus n_user plus two per cell:
+// every cell's count is below n_user E_c / E_tot + 1 per source kind).
+int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uint32_t rank,
+                        double dt, double next_dt, bh_resident_report* rep,
+                        bh_instance_report* irep) {
+  const uint32_t nc = h->n_cells, K = h->n_inst;
+  uint64_t n_user_total = 0;
+  for (uint32_t k = 0; k < (K ? K : 1u); ++k) n_user_total += n_user[k];
+  const uint64_t bound = h->n_cen + n_user_total + 2 * (uint64_t)nc;
+  if (bound >= (uint64_t)kHole)
+    return fail(h, BH_ERR_CAPACITY, "up to %llu photons in one step: at most %u are supported",
+                (unsigned long long)bound, kHole - 1u);
+  const uint64_t slack = census_sizes(h, 0).slots;
+  TRY(grow_store(h, h->W, bound + slack, h->n_cen));
+  TRY(grow_store(h, h->C, bound + slack, 0));
+  const uint64_t cap = std::min(h->W.cap, h->C.cap) - slack;
+  const CensusSizes z = census_sizes(h, cap);
+  const uint64_t pc = photon_cap(h, cap);
+  const uint32_t nb = resident_blocks(h);
+  TRY(ensure(h, h->counts, nc));
+  TRY(ensure(h, h->off, std::max<uint64_t>(nc, pc)));
+  TRY(ensure(h, h->cflag, pc));
+  TRY(ensure(h, h->hrec, z.rec_cap));
+  TRY(ensure(h, h->hole_len, z.rec_cap));
+  TRY(ensure(h, h->hole_off, z.rec_cap));
+  TRY(ensure(h, h->tail_live, z.hole_cap));
+  TRY(ensure(h, h->tail_rank, z.hole_cap));
+  TRY(ensure(h, h->sum_part, (size_t)8 * std::max<uint32_t>(nb, kSumMaxBlocks)));
+  const uint64_t scan_max = std::max<uint64_t>(std::max<uint64_t>(nc, pc),
+                                               std::max(z.hole_cap, z.rec_cap));
+  TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(scan_max, kScanTile), 1)));
+  TRY(ensure(h, h->tile_E, std::max<uint64_t>(div_up(scan_max, kScanTile), 1)));
+  if (h->opt.precision == BH_PRECISION_FP32_TRACK) {
+    const uint64_t r = std::max(h->W.cap, h->C.cap);
+    TRY(ensure(h, h->rep_idx, r));
+    TRY(ensure(h, h->rep_first, r));
+    TRY(ensure(h, h->rep_E, r));
+  }
+  if (!K) {  // one problem: the two source records live in pinned memory too
+    if (h->h_inst_cap < 1) {
+      HIPCHK(h, hipHostMalloc(&h->h_rec, 2 * sizeof(SourceInst), hipHostMallocDefault));
+      HIPCHK(h, hipHostMalloc(&h->h_inst_rep, sizeof(InstReport), hipHostMallocDefault));
+      h->h_inst_cap = 1;
+    }
+  }
+  resident_records(h, n_user, step, rank, h->h_rec);
+  *h->h_params = StepParams{h->n_cen, 0, 0, kC * next_dt, dt, SourceInst{}, cap, 0u, 0u};
+  bh_handle::StepGraph& g = h->resident_graphs[h->W.slab < h->C.slab ? 0 : 1];
+  const std::vector<uintptr_t> key = resident_graph_key(h, n_user_total);
+  if (!g.exec || g.key != key) {
+    if (g.exec) HIPCHK(h, hipGraphExecDestroy(g.exec));
+    if (g.graph) HIPCHK(h, hipGraphDestroy(g.graph));
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.key.clear();
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_resident(h, cap, n_user[0]);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (rc != BH_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess)
+      return fail(h, BH_ERR_DEVICE, "capturing the resident step failed: %s",
+                  hipGetErrorString(e));
+    g.graph = graph;
+    HIPCHK(h, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    g.key = key;
+  }
+  HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+  HIPCHK(h, hipGraphLaunch(g.exec, h->stream));
+  HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const char* hp = static_cast<const char*>(h->h_pinned);
+  CensusPartial tot;
+  DevCounters ctrs;
+  PhysTotals pt;
+  StepParams Pd;
+  std::memcpy(&tot, hp, sizeof tot);
+  std::memcpy(&ctrs, hp + 256, sizeof ctrs);
+  std::memcpy(&pt, hp + 2048, sizeof pt);
+  std::memcpy(&Pd, hp + 3072, sizeof Pd);
+  if (ctrs.bad_args || tot.overflow || Pd.overflow) {
+    h->n_cen = 0;
+    h->n_new = 0;
+    return fail(h, BH_ERR_DEVICE,
+                ctrs.bad_args ? "transport kernel argument check failed; nothing transported"
+                              : "resident step: photon counts exceeded the stores' capacity");
+  }
+  if (!K && !(pt.global_E > 0.0)) return fail(h, BH_ERR_STATE, "no photon energy this step");
+  std::swap(h->W, h->C);
+  h->n_cen = tot.n_census;
+  h->n_new = 0;
+  h->last_source_ms = 0.0;
+  if (!K && (pt.ambiguous || h->mat_ranks > 1)) ++h->serial_etot_steps;
+  bh_step_stats st;
+  std::memset(&st, 0, sizeof st);
+  st.census_E = tot.census_E;
+  st.exit_E = tot.exit_E;
+  st.n_transported = Pd.n;
+  st.n_census = tot.n_census;
+  st.n_exit = tot.n_exit;
+  st.n_killed = tot.n_killed;
+  st.n_segments = ctrs.segments;
+  st.n_flushes = ctrs.flushes;
+  st.n_replayed = ctrs.replays;
+  st.lattice = h->lattice ? 1 : 0;
+  st.tally = h->last_tally;
+  st.transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph
+  fill_report(h, pt, st, Pd.n, Pd.n_new, false, rep);
+  if (K) fill_instance_reports(h, h->h_inst_rep, rep, irep);
+  if (rep) rep->graph = 1;
+  if (ctrs.stuck)
+    return fail(h, BH_ERR_DEVICE, "%llu photon(s) exceeded %u segments and were stopped",
+                ctrs.stuck, kMaxSegments);
+  if (tot.n_other)
+    return fail(h, BH_ERR_STATE, "%llu photon(s) ended without EXIT/CENSUS/KILLED",
+                tot.n_other);
+  return BH_OK;
+}
+
+int resident_checks(bh_handle* h, uint32_t step, double dt) {
+  if (!h->has_mesh || !h->has_material)
+    return fail(h, BH_ERR_STATE, "a resident step needs bh_set_mesh and bh_set_material");
+  if (step == 0) return fail(h, BH_ERR_ARG, "step counts from 1");
+  if (!(dt > 0.0)) return fail(h, BH_ERR_ARG, "dt must be > 0");
+  return BH_OK;
+}
+
+// Steps after the first run as one graph (persistent algorithm).
+bool resident_uses_graph(const bh_handle* h, uint32_t step) {
+  return step > 1 && h->opt.algorithm == BH_ALGO_PERSISTENT && !h->has_census_E;
+}
+
+}  // namespace
+
+extern "C" {
+
 int bh_step_resident(bh_handle* h, uint64_t n_user_photons, uint32_t seed, uint32_t step,
                      uint32_t rank, double dt, double next_dt, bh_resident_report* rep) {
   if (!h || !h->stream) return BH_ERR_ARG;
-  if (!h->has_mesh || !h->has_material)
-    return fail(h, BH_ERR_STATE, "bh_step_resident needs bh_set_mesh and bh_set_material");
-  if (h->n_inst) return fail(h, BH_ERR_STATE, "resident step physics runs one problem");
-  if (step == 0) return fail(h, BH_ERR_ARG, "step counts from 1");
+  TRY(resident_checks(h, step, dt));
+  if (h->n_inst)
+    return fail(h, BH_ERR_STATE, "the mesh holds %u instances: use bh_step_resident_instances",
+                h->n_inst);
   if (n_user_photons == 0 || n_user_photons > 0xFFFFFFFFull)
     return fail(h, BH_ERR_ARG, "n_user_photons must be in [1, 2^32)");
-  if (!(dt > 0.0)) return fail(h, BH_ERR_ARG, "dt must be > 0");
   HIPCHK(h, hipSetDevice(h->device));
   // the step's physics fills every group alike (Cell::set_op_a / set_op_s)
   h->uniform_sigma_a = true;
@@ -2030,106 +2505,26 @@ int bh_step_resident(bh_handle* h, uint64_t n_user_photons, uint32_t seed, uint3
   if (h->n_cen && step > 1 && ctr_hi != h->ctr_hi)
     return fail(h, BH_ERR_ARG, "seed differs from the resident census photons' seed");
   h->ctr_hi = ctr_hi;
-  const uint32_t n = h->n_cells;
-  const bool first = step == 1;
-  const MatDev md = mat_dev(h);
-  // Mesh::calculate_photon_energy
-  k_phys_cells<<<grid_for(n, 256), 256, 0, h->stream>>>(
-      h->geo.p, h->sigma.p, h->lattice ? h->rec.p : nullptr, h->G, n, md, dt, h->mat_rf,
-      first ? 1 : 0, h->em.p, h->src.p, h->cen.p, h->mat_e.p);
-  HIPCHK(h, hipGetLastError());
-  const uint64_t rank_off = n_user_photons * (uint64_t)rank;
-  const SourceInst host2[2] = {
-      SourceInst{n_user_photons, 0.0, rank_off, ctr_hi, 0, 0},
-      SourceInst{n_user_photons, 0.0, 10000000000000ull * (uint64_t)step + rank_off, ctr_hi, 0,
-                 0}};
-  char* hp = static_cast<char*>(h->h_pinned);
-  std::memcpy(hp + 1024, host2, sizeof host2);
-  HIPCHK(h, hipMemcpyAsync(h->d_inst2, hp + 1024, sizeof host2, hipMemcpyHostToDevice,
-                           h->stream));
-  HIPCHK(h, hipMemsetAsync(h->d_phys, 0, sizeof(PhysTotals), h->stream));
-  const uint32_t nb = std::max<uint32_t>(1u, std::min<uint32_t>(kSumMaxBlocks, div_up(n, 4096)));
-  TRY(ensure(h, h->sum_part, (size_t)8 * nb));
-  const bool serial = h->mat_ranks > 1;
-  if (!serial) {
-    // E_tot in parallel with a bound on the serial loop's value, and a check
-    // that no photon count depends on where in the bound it lies
-    k_phys_partials<<<nb, kSumNT, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
-                                                  h->sum_part.p);
-    k_final_sums<7><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->tot_emission);
-    k_phys_etot<<<1, 1, 0, h->stream>>>(n, h->d_phys, h->d_inst2);
-    k_count_check<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, n,
-                                                           n_user_photons, first ? 1 : 0,
-                                                           h->d_phys);
-    HIPCHK(h, hipGetLastError());
-  }
-  PhysTotals pt;
-  HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  std::memcpy(&pt, hp + 2048, sizeof pt);
-  const bool used_serial = pt.ambiguous != 0;
-  if (serial || used_serial) {  // the reference's serial cell-order sums
-    k_phys_totals<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
-                                              h->mat_ranks, 0, h->d_phys, h->d_inst2);
-    HIPCHK(h, hipGetLastError());
-    if (h->mat_ranks > 1) {  // with one rank the reassignment leaves every energy as it is
-      k_phys_rank_sums<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, n,
-                                                   n_user_photons, h->mat_ranks, h->mat_rf,
-                                                   first ? 1 : 0, h->d_phys, h->d_inst2);
-      k_phys_reassign<<<grid_for(n, 256), 256, 0, h->stream>>>(
-          h->em.p, h->src.p, h->cen.p, n, n_user_photons, h->mat_ranks, h->mat_rank, h->mat_rf,
-          first ? 1 : 0, h->d_phys);
-      k_phys_totals<<<1, kWave, 0, h->stream>>>(h->em.p, h->src.p, h->cen.p, h->mat_e.p, n,
-                                                h->mat_ranks, 1, h->d_phys, h->d_inst2);
-      HIPCHK(h, hipGetLastError());
-    }
-    HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost,
-                             h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    std::memcpy(&pt, hp + 2048, sizeof pt);
-    ++h->serial_etot_steps;
-  }
-  if (!(pt.global_E > 0.0)) return fail(h, BH_ERR_STATE, "no photon energy this step");
-  h->src_neff_frac = pt.e_sq > 0.0 ? (pt.e_sum * pt.e_sum / pt.e_sq) / (double)n : 1.0;
-  h->has_energy = true;
-  h->has_census_E = first;
-  // source: initial census at step 1, then this step's photons
-  uint64_t n_new = 0;
-  TRY(run_source(h, host2[0], host2[1], h->d_inst2, h->d_inst2 + 1, dt, &n_new,
-                 first ? h->sum_part.p : nullptr, nb));
-  if (first) {
-    k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->pre_census);
-    HIPCHK(h, hipGetLastError());
-  }
-  const uint64_t n_transported = h->n_cen + n_new;
-  // transport + census; the tallies stay on the device
-  bh_step_stats st;
-  TRY(bh_transport(h, next_dt, nullptr, nullptr, &st));
-  // Mesh::update_temperature
-  k_update_temperature<<<grid_for(n, 256), 256, 0, h->stream>>>(
-      h->geo.p, md, h->abs_E.p, h->track_E.p, h->em.p, dt, n, h->tr_diag.p, h->mat_e.p);
-  HIPCHK(h, hipGetLastError());
-  k_report_partials<<<nb, kSumNT, 0, h->stream>>>(h->abs_E.p, h->mat_e.p, n, h->sum_part.p);
-  k_final_sums<2><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->absorbed);
-  HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipMemcpyAsync(hp + 2048, h->d_phys, sizeof pt, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  std::memcpy(&pt, hp + 2048, sizeof pt);
-  if (rep) {
-    std::memset(rep, 0, sizeof *rep);
-    rep->total_E = pt.global_E;
-    rep->emission_E = pt.tot_emission;
-    rep->source_E = pt.tot_source;
-    rep->pre_census_E = first ? pt.pre_census : 0.0;
-    rep->pre_mat_E = pt.pre_mat;
-    rep->post_mat_E = pt.post_mat;
-    rep->absorbed_E = pt.absorbed;
-    rep->stats = st;
-    rep->stats.n_transported = n_transported;
-    rep->n_new = n_new;
-    rep->etot_serial = (serial || used_serial) ? 1 : 0;
-  }
-  return BH_OK;
+  if (resident_uses_graph(h, step))
+    return resident_step_graph(h, &n_user_photons, step, rank, dt, next_dt, rep, nullptr);
+  return resident_step_eager(h, &n_user_photons, step, rank, dt, next_dt, rep, nullptr);
+}
+
+int bh_step_resident_instances(bh_handle* h, const uint64_t* n_user_photons, uint32_t step,
+                               uint32_t rank, double dt, double next_dt,
+                               bh_resident_report* total, bh_instance_report* per_instance) {
+  if (!h || !h->stream || !n_user_photons) return BH_ERR_ARG;
+  TRY(resident_checks(h, step, dt));
+  if (!h->n_inst) return fail(h, BH_ERR_STATE, "no instances: call bh_set_instances");
+  for (uint32_t k = 0; k < h->n_inst; ++k)
+    if (n_user_photons[k] == 0 || n_user_photons[k] > 0xFFFFFFFFull)
+      return fail(h, BH_ERR_ARG, "instance %u: n_user_photons must be in [1, 2^32)", k);
+  HIPCHK(h, hipSetDevice(h->device));
+  h->uniform_sigma_a = true;
+  h->gray_sigma_a = h->opt.tally_mode != 3;
+  if (resident_uses_graph(h, step))
+    return resident_step_graph(h, n_user_photons, step, rank, dt, next_dt, total, per_instance);
+  return resident_step_eager(h, n_user_photons, step, rank, dt, next_dt, total, per_instance);
 }
 
 int bh_get_temperature(bh_handle* h, double* T_e, double* T_r) {

@@ -69,6 +69,9 @@ struct CensusOut {
   uint64_t n_cen, n_new;
   uint64_t slots, rec_cap;  // capacity of C, of rec
   int on;
+  // several independent problems (else null): exit energy per instance,
+  // added by every history that leaves through a vacuum or source face
+  double* inst_exit;
 };
 
 struct DevCells {
@@ -115,6 +118,12 @@ struct StepParams {
   double census_life;   // c * next_dt
   double dt;
   SourceInst fresh;     // one problem's source parameters
+  // a resident step (physics on the device) captured as a graph: the photon
+  // counts are only known on the device, so n_new and n are set there, from
+  // the source counts, within the capacity the stores were sized for
+  uint64_t cap;         // photons W and C hold (n is clamped to it)
+  uint32_t overflow;    // set when the counts exceeded cap (never, by sizing)
+  uint32_t pad;
 };
 
 struct DevCounters {

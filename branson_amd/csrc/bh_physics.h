@@ -198,6 +198,7 @@ __device__ __forceinline__ double face_area(const CellGeo& g, int face) {
 // Per cell: opacities and Fleck factor into the tracking records (every
 // group, Cell::set_op_a / set_op_s, src/cell.h:260-275), and the emission,
 // initial-census and face-source energies (src/mesh.h:257-281).
+// With P (a step captured as a graph), dt is the step's, read on the device.
 __global__ __launch_bounds__(256) void k_phys_cells(CellGeo* __restrict__ geo,
                                                     double2* __restrict__ sigma,
                                                     CellRec* __restrict__ rec, uint32_t G,
@@ -205,9 +206,11 @@ __global__ __launch_bounds__(256) void k_phys_cells(CellGeo* __restrict__ geo,
                                                     int first_step, double* __restrict__ em,
                                                     double* __restrict__ src,
                                                     double* __restrict__ cen,
-                                                    double* __restrict__ mat) {
+                                                    double* __restrict__ mat,
+                                                    const StepParams* __restrict__ P) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (P) dt = P->dt;
   const CellGeo g = geo[i];
   const double vol = cell_volume(g);
   const double cV = m.cv[i], T = m.T_e[i], Tr = m.T_r[i], Ts = m.T_s[i], rho = m.rho[i];
@@ -239,13 +242,17 @@ __global__ __launch_bounds__(256) void k_phys_cells(CellGeo* __restrict__ geo,
 // reassignment (n_ranks equal values summed in rank order), after which
 // k_phys_rank_sums sets E_tot and k_phys_reassign applies this rank's
 // share.  Pass 2 (pass = 1) recomputes this rank's step totals only.
+// only_if_ambiguous: run only when k_count_check flagged a count (the
+// serial fallback inside a step that no host round trip splits).
 __global__ __launch_bounds__(kWave) void k_phys_totals(const double* __restrict__ em,
                                                        const double* __restrict__ src,
                                                        const double* __restrict__ cen,
                                                        const double* __restrict__ mat,
                                                        uint32_t n, uint32_t n_ranks, int pass,
                                                        PhysTotals* __restrict__ out,
-                                                       SourceInst* __restrict__ inst2) {
+                                                       SourceInst* __restrict__ inst2,
+                                                       int only_if_ambiguous) {
+  if (only_if_ambiguous && out->ambiguous == 0u) return;
   double t_em = 0.0, t_cen = 0.0, t_src = 0.0, total = 0.0, pre_mat = 0.0, e1 = 0.0, e2 = 0.0;
 #pragma unroll 8
   for (uint32_t c = 0; c < n; ++c) {
@@ -347,15 +354,139 @@ __global__ __launch_bounds__(256) void k_phys_reassign(double* __restrict__ em,
   src[c] = s;
 }
 
+// A step captured as a graph: the step's photon counts from the source
+// scan's total, within the stores' capacity.
+__global__ void k_step_counts(StepParams* __restrict__ P, const uint64_t* __restrict__ n_new) {
+  const uint64_t room = P->cap > P->n_cen ? P->cap - P->n_cen : 0;
+  const uint64_t k = *n_new;
+  P->n_new = k < room ? k : room;
+  P->n = P->n_cen + P->n_new;
+  P->overflow = k > room ? 1u : 0u;
+}
+
+// ---- several independent problems (instances) ------------------------------
+// Per-instance step totals (the batched workload of many problems, which has
+// no counterpart in the reference): one wave per instance sums its cells in
+// a fixed order (lane-strided partial sums, then a fixed butterfly), so a
+// run is reproducible.  An instance's photon energy total_E is the sum of
+// its cells' source + census + emission energies; its photon counts
+// int(n_user E_c / total_E) follow from it.
+struct InstReport {  // bh_instance_report, field for field
+  double total_E, emission_E, source_E, pre_census_E, post_census_E;
+  double absorbed_E, exit_E, pre_mat_E, post_mat_E;
+  uint64_t n_new;
+};
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+  return v;
+}
+
+// After the physics of the step: energies, material energy and total_E per
+// instance; total_E goes into the instance's source records (census, new).
+__global__ __launch_bounds__(kWave) void k_inst_sums(const double* __restrict__ em,
+                                                     const double* __restrict__ src,
+                                                     const double* __restrict__ cen,
+                                                     const double* __restrict__ mat,
+                                                     const uint32_t* __restrict__ first_cell,
+                                                     uint32_t K, InstReport* __restrict__ rep,
+                                                     SourceInst* __restrict__ rec) {
+  const uint32_t k = blockIdx.x;
+  if (k >= K) return;
+  double e = 0.0, s = 0.0, m = 0.0, t = 0.0;
+  for (uint32_t i = first_cell[k] + threadIdx.x; i < first_cell[k + 1]; i += kWave) {
+    const double ei = em[i], si = src[i], ci = cen[i];
+    e += ei;
+    s += si;
+    m += mat[i];
+    t += si + ci + ei;
+  }
+  e = wave_sum_f64(e);
+  s = wave_sum_f64(s);
+  m = wave_sum_f64(m);
+  t = wave_sum_f64(t);
+  if (threadIdx.x != 0) return;
+  InstReport& r = rep[k];
+  r.total_E = t;
+  r.emission_E = e;
+  r.source_E = s;
+  r.pre_census_E = 0.0;  // the census photons' energy, summed once they exist
+  r.pre_mat_E = m;
+  r.post_census_E = 0.0;
+  r.exit_E = 0.0;
+  rec[k].total_E = t;
+  rec[K + k].total_E = t;
+}
+
+// Census energy per instance: the photons of W[0, *n) (or [0, n)) summed
+// into out[instance] (post = 1: post_census_E, else pre_census_E).  Atomic
+// adds, each wave's adds combined when all its photons are in one instance.
+__global__ __launch_bounds__(256) void k_inst_census(const uint32_t* __restrict__ cell,
+                                                     const double* __restrict__ E, uint64_t n,
+                                                     const unsigned long long* __restrict__ n_ptr,
+                                                     const uint64_t* __restrict__ n_ptr64,
+                                                     const uint32_t* __restrict__ cell_inst,
+                                                     InstReport* __restrict__ rep, int post) {
+  if (n_ptr) n = *n_ptr < n ? *n_ptr : n;
+  if (n_ptr64) n = *n_ptr64 < n ? *n_ptr64 : n;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += stride) {
+    const uint64_t i = b + threadIdx.x;
+    const bool in = i < n;
+    const uint32_t k = in ? cell_inst[cell[i]] : 0u;
+    const double e = in ? E[i] : 0.0;
+    const uint32_t k0 = (uint32_t)__shfl((int)k, 0, kWave);
+    const bool same = __ballot(in && k != k0) == 0;
+    double* dst = post ? &rep[0].post_census_E : &rep[0].pre_census_E;
+    const size_t stride_d = sizeof(InstReport) / sizeof(double);
+    if (same) {
+      const double w = wave_sum_f64(e);
+      if ((threadIdx.x & (kWave - 1)) == 0 && w != 0.0) atomicAdd(dst + k0 * stride_d, w);
+    } else if (in) {
+      atomicAdd(dst + k * stride_d, e);
+    }
+  }
+}
+
+// After update_temperature: absorbed and material energy, new photons and
+// exit energy per instance.
+__global__ __launch_bounds__(kWave) void k_inst_report(const double* __restrict__ abs_E,
+                                                       const double* __restrict__ mat,
+                                                       const uint64_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ first_cell,
+                                                       const double* __restrict__ inst_exit,
+                                                       uint32_t K, InstReport* __restrict__ rep) {
+  const uint32_t k = blockIdx.x;
+  if (k >= K) return;
+  double a = 0.0, m = 0.0;
+  unsigned long long c = 0;
+  for (uint32_t i = first_cell[k] + threadIdx.x; i < first_cell[k + 1]; i += kWave) {
+    a += abs_E[i];
+    m += mat[i];
+    c += counts[i];
+  }
+  a = wave_sum_f64(a);
+  m = wave_sum_f64(m);
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
+  if (threadIdx.x != 0) return;
+  rep[k].absorbed_E = a;
+  rep[k].post_mat_E = m;
+  rep[k].n_new = c;
+  rep[k].exit_E = inst_exit[k];
+}
+
 // Mesh::update_temperature, one rank (src/mesh.h:327-423): new electron
 // temperature from the step's absorbed and emitted energy, the diagnostic
 // radiation temperature from the track tally, and T cV V rho per cell.
 __global__ __launch_bounds__(256) void k_update_temperature(
     const CellGeo* __restrict__ geo, MatDev m, const double* __restrict__ abs_E,
     const double* __restrict__ track_E, const double* __restrict__ em, double dt, uint32_t n,
-    double* __restrict__ T_r_out, double* __restrict__ mat) {
+    double* __restrict__ T_r_out, double* __restrict__ mat, const StepParams* __restrict__ P) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (P) dt = P->dt;
   const double cV = m.cv[i], rho = m.rho[i], vol = cell_volume(geo[i]), T = m.T_e[i];
   const double T_new = T + (abs_E[i] - em[i]) / (cV * vol * rho);
   T_r_out[i] = pow(track_E[i] / (vol * dt * kA * kC), 0.25);

@@ -1035,6 +1035,10 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
   // pointer-selected increment that puts them in scratch memory.
   const bool ex = ended && L.desc == kExit, kl = ended && L.desc == kKilled;
   loc.exit_E += ex ? (double)L.E : 0.0;
+  if (ex && cells.cell_inst) {  // several problems: each instance's exit energy
+    const CensusOut co = arg_census();
+    if (co.inst_exit) atomicAdd(&co.inst_exit[cells.cell_inst[L.cell]], (double)L.E);
+  }
   loc.n_exit += (uint32_t)__popcll(__ballot(ex));
   loc.n_killed += (uint32_t)__popcll(__ballot(kl));
   loc.n_other += (uint32_t)__popcll(__ballot(ended && !cen && !ex && !kl));

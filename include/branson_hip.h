@@ -338,7 +338,7 @@ typedef struct bh_resident_report {
                              than one rank, or a photon count that a
                              parallel sum's last bits could have changed);
                              0: parallel sum, every count proven equal     */
-  int32_t pad;
+  int32_t graph;          /* 1: the step ran as one HIP graph launch       */
 } bh_resident_report;
 
 /* Uploads the material (replaces any earlier one).  replicated_factor,
@@ -353,9 +353,32 @@ int bh_set_material(bh_handle *h, const bh_material *m, double replicated_factor
  * (initial census at step 1), transport, census and Mesh::update_temperature
  * (src/mesh.h:327-423).  Temperatures, opacities, energies and tallies stay
  * on the device; only the report crosses to the host.  `step` is the
- * reference's cycle number (1 = first step: initial census). */
+ * reference's cycle number (1 = first step: initial census).  With the
+ * persistent algorithm every step after the first is one HIP graph launch
+ * with no host round trip inside it (the photon stores are sized before it
+ * for the most photons the step can make). */
 int bh_step_resident(bh_handle *h, uint64_t n_user_photons, uint32_t seed, uint32_t step,
                      uint32_t rank, double dt, double next_dt, bh_resident_report *rep);
+
+/* Several independent problems (bh_set_instances) stepped with their physics
+ * on the device (no counterpart in the reference: the batched workload).
+ * The material of bh_set_material holds every instance's cells (one rank,
+ * replicated_factor 1); instance k sources n_user_photons[k] photons with
+ * its own seed, and its photon energy E_tot is the sum of its cells'
+ * energies, summed in a fixed order.  total (may be NULL) receives the
+ * sums over instances and the step's transport stats; per_instance (may be
+ * NULL) one record per instance.  Steps after the first run as one HIP
+ * graph, as bh_step_resident's. */
+typedef struct bh_instance_report {
+  double total_E, emission_E, source_E;
+  double pre_census_E, post_census_E; /* census photons' energy before/after  */
+  double absorbed_E, exit_E;
+  double pre_mat_E, post_mat_E;
+  uint64_t n_new;                     /* photons the instance sourced          */
+} bh_instance_report;
+int bh_step_resident_instances(bh_handle *h, const uint64_t *n_user_photons, uint32_t step,
+                               uint32_t rank, double dt, double next_dt,
+                               bh_resident_report *total, bh_instance_report *per_instance);
 
 /* Device-resident state after bh_step_resident: electron temperature and
  * the diagnostic radiation temperature (T_r = (track_E / (V dt a c))^1/4),
