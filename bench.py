@@ -305,8 +305,10 @@ def _bench_headline(a, capi, cube=True):
     }
     graph = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT,
                       step_graph=True, **kw)
-    resident = _run_resident(capi, a.steps, a.warmup, deck=HOHLRAUM,
-                             n_photons=HOHLRAUM_PHOTONS, n_groups=30, census_steps=1)
+    event = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_EVENT, **kw)
+    resident = None if a.no_resident else _run_resident(
+        capi, a.steps, a.warmup, deck=HOHLRAUM, n_photons=HOHLRAUM_PHOTONS, n_groups=30,
+        census_steps=1)
     bytes_launch, kms, achieved = _roofline("fp64", plain)
     line = _common(
         METRIC, ch_rate, a.steps, a.warmup, 1e3 * ch["dev_s"] / a.steps, "f64",
@@ -325,6 +327,11 @@ def _bench_headline(a, capi, cube=True):
         fp32_tracking=fp32_block,
         graph_steps={"histories_per_s": graph["hist"] / graph["dev_s"],
                      "ms_per_step": 1e3 * graph["dev_s"] / a.steps},
+        event_form={"form": "plain fp64, event-split passes over the compacted list of "
+                            "live photons (same events)",
+                    "histories_per_s": event["hist"] / event["dev_s"],
+                    "ms_per_step": 1e3 * event["dev_s"] / a.steps,
+                    "kernel_ms_per_step": _kernel_split(event)},
         whole_step_histories_per_s=plain["hist"] / plain["step_s"],
         resident_whole_step_histories_per_s=resident,
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -340,6 +347,7 @@ def _bench_headline(a, capi, cube=True):
     if cube:
         c = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_PERSISTENT)
         cb, cms, cach = _roofline("fp64", c)
+        ce = _run_form(capi, a.steps, a.warmup, capi.PRECISION_FP64, capi.ALGO_EVENT)
         line["cube_throughput"] = {
             "workload": CUBE_WORKLOAD, "form": "plain fp64, persistent waves",
             "histories_per_s": c["hist"] / c["dev_s"], "segments_per_s": c["segs"] / c["dev_s"],
@@ -348,7 +356,10 @@ def _bench_headline(a, capi, cube=True):
             "kernel_ms_per_step": _kernel_split(c),
             "transport_kernel_roofline": {"achieved_GBps": cach, "frac": cach / HBM_PEAK_GBPS,
                                           "kernel_ms_mean": cms,
-                                          "algorithmic_bytes_per_launch": cb}}
+                                          "algorithmic_bytes_per_launch": cb},
+            "event_form": {"histories_per_s": ce["hist"] / ce["dev_s"],
+                           "ms_per_step": 1e3 * ce["dev_s"] / a.steps,
+                           "kernel_ms_per_step": _kernel_split(ce)}}
     line["cpu_baseline"] = None if a.no_cpu else _cpu_baseline(
         HOHLRAUM, 30, HOHLRAUM_PHOTONS, 1,
         "step 1 of the same 30-group hohlraum deck at 5e7 photons/step")
@@ -364,9 +375,12 @@ def _bench_batched(a, capi):
 
     steps = min(a.steps, 3)
     res, step1 = {}, {}
-    for prec in (capi.PRECISION_FP64, capi.PRECISION_FP32_TRACK):
+    for prec, algo in ((capi.PRECISION_FP64, capi.ALGO_PERSISTENT),
+                       (capi.PRECISION_FP32_TRACK, capi.ALGO_PERSISTENT),
+                       (capi.PRECISION_FP64, capi.ALGO_EVENT)):
         b = problems.batched_instances()
-        with capi.Engine(0, algorithm=capi.ALGO_PERSISTENT, precision=prec,
+        key = prec if algo == capi.ALGO_PERSISTENT else "event"
+        with capi.Engine(0, algorithm=algo, precision=prec,
                          fp32_tol=a.fp32_tol,
                          reserve_photons=int(b.n_photons.sum()) * 3) as e:
             run = BatchedRun(b, e)
@@ -374,8 +388,8 @@ def _bench_batched(a, capi):
             for k in range(steps):
                 reps.append(run.step())
                 if k == 0:
-                    step1[prec] = run.tallies()[0]
-        res[prec] = reps
+                    step1[key] = run.tallies()[0]
+        res[key] = reps
 
     def rate(reps):
         return sum(r["n_transported"] for r in reps) / sum(r["device_s"] for r in reps)
@@ -398,6 +412,7 @@ def _bench_batched(a, capi):
                     "histories": sum(r["n_transported"] for r in p64),
                     "segments": sum(r["n_segments"] for r in p64)},
                    max_instance_balance=max(r["max_instance_balance"] for r in p64),
+                   event_form={"histories_per_s": rate(res["event"])},
                    chosen=chosen)
 
 
@@ -476,6 +491,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-resident", action="store_true",
+                    help="skip the resident whole-step leg (the other legs' launches are "
+                         "unchanged; profiling runs use it)")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "cube", "batched", "adjoint"],
                     help="headline (default): the 30-group hohlraum, plain FP64 against "

@@ -21,7 +21,7 @@ ERRORS = {-1: "BH_ERR_ARG", -2: "BH_ERR_DEVICE", -3: "BH_ERR_ALLOC", -4: "BH_ERR
 REFLECT, VACUUM, ELEMENT, SOURCE, PROCESSOR = range(5)
 EXIT, PASS, CENSUS, SCATTER, KILLED, BOUND = range(6)
 PRECISION_FP64, PRECISION_FP32_TRACK = 0, 1
-ALGO_HISTORY, ALGO_PERSISTENT = 0, 1
+ALGO_HISTORY, ALGO_PERSISTENT, ALGO_EVENT = 0, 1, 2
 
 # the reference's 120-byte AOS Photon (src/photon.h:172-181)
 PHOTON_DTYPE = np.dtype([

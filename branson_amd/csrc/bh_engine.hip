@@ -191,6 +191,12 @@ struct bh_handle {
   SourceInst* h_rec = nullptr;          // pinned: the step's source records
   InstReport* h_inst_rep = nullptr;     // pinned: per-instance totals read back
   size_t h_inst_cap = 0;                // instances the two pinned buffers hold
+  // event-split form: open cell visits, segment counts, last faces and the
+  // active / scatter lists, one entry per photon slot
+  DArr<double> ev_abs, ev_track;
+  DArr<uint32_t> ev_segs, ev_list0, ev_list1, ev_scatter;
+  DArr<uint8_t> ev_sc;
+  unsigned long long* d_ev_count = nullptr;
   DArr<uint64_t> rep_idx;    // FP32 form: histories handed to the FP64 replay
   DArr<uint32_t> rep_first;
   DArr<double> rep_E;
@@ -540,6 +546,60 @@ int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const
   return BH_OK;
 }
 
+// Event-split transport of W[0, n): flight and scatter passes over the
+// compacted list of live photons until it is empty.  The host reads the live
+// count every few pass pairs (the counts only fall, so the last one read
+// bounds the grids of the pairs in between).
+int launch_event(bh_handle* h, const PhotonSoA& W, uint64_t n, const CensusOut& co,
+                 const Sinks& s, bool lds) {
+  const uint64_t cap = std::max<uint64_t>({n, h->W.cap, h->C.cap, 1});
+  if (cap >= (uint64_t)0xFFFFFFFFu)
+    return fail(h, BH_ERR_CAPACITY, "the event-split lists index photons with 32 bits");
+  TRY(ensure(h, h->ev_abs, cap));
+  TRY(ensure(h, h->ev_track, cap));
+  TRY(ensure(h, h->ev_segs, cap));
+  TRY(ensure(h, h->ev_sc, cap));
+  TRY(ensure(h, h->ev_list0, cap));
+  TRY(ensure(h, h->ev_list1, cap));
+  TRY(ensure(h, h->ev_scatter, cap));
+  const EventState ev{h->ev_abs.p,   h->ev_track.p, h->ev_segs.p, h->ev_sc.p,
+                      {h->ev_list0.p, h->ev_list1.p}, h->ev_scatter.p, h->d_ev_count};
+  const DevCells cells = dev_cells(h);
+  k_ev_init<<<grid_for(n, 256), 256, 0, h->stream>>>(ev, n);
+  HIPCHK(h, hipGetLastError());
+  const unsigned max_grid = (unsigned)h->n_cu * 16;
+  uint64_t live = n;
+  int cur = 0;
+  unsigned long long* hp = reinterpret_cast<unsigned long long*>(
+      static_cast<char*>(h->h_pinned) + 3584);
+  return with_tally(h, lds, s, [&](auto t, size_t shm) -> int {
+    using T = decltype(t);
+    while (live > 0) {
+      for (int k = 0; k < 8; ++k) {
+        const unsigned grid = std::min<unsigned>(grid_for(live, 256), max_grid);
+        k_ev_clear<<<1, 1, 0, h->stream>>>(ev, cur ^ 1);
+        if (h->lattice)
+          k_ev_flight<T, true><<<grid, 256, shm, h->stream>>>(W, co, cells, ev, cur, s.ctr_hi, t,
+                                                               s.ctrs);
+        else
+          k_ev_flight<T, false><<<grid, 256, shm, h->stream>>>(W, co, cells, ev, cur, s.ctr_hi,
+                                                                t, s.ctrs);
+        if (h->lattice)
+          k_ev_scatter<true><<<grid, 256, 0, h->stream>>>(W, cells, ev, cur, s.ctr_hi);
+        else
+          k_ev_scatter<false><<<grid, 256, 0, h->stream>>>(W, cells, ev, cur, s.ctr_hi);
+        HIPCHK(h, hipGetLastError());
+        cur ^= 1;
+      }
+      HIPCHK(h, hipMemcpyAsync(hp, &h->d_ev_count[cur], sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      live = *hp;
+    }
+    return BH_OK;
+  });
+}
+
 // Default persistent grid: the kernel's resident blocks per CU, at most this.
 constexpr int kMaxBlocksPerCu = 8;
 
@@ -567,6 +627,10 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
     TRY(ensure(h, h->rep_first, cap));
     TRY(ensure(h, h->rep_E, cap));
     rl = ReplayList{h->rep_idx.p, h->rep_first.p, h->rep_E.p};
+  }
+  if (h->opt.algorithm == BH_ALGO_EVENT) {
+    if (P) return fail(h, BH_ERR_STATE, "the event-split algorithm is not captured as a graph");
+    return launch_event(h, W, n, co, s, lds);
   }
   if (!persistent) {
     return with_tally(h, lds, s, [&](auto t, size_t shm) -> int {
@@ -662,6 +726,12 @@ CensusSizes census_sizes(const bh_handle* h, uint64_t n) {
   const bool fp32 = h->opt.precision == BH_PRECISION_FP32_TRACK;
   const uint64_t w_rep = fp32 ? (uint64_t)h->n_cu * 4 * 256 / kWave : 0;
   CensusSizes z;
+  if (h->opt.algorithm == BH_ALGO_EVENT) {  // dense census writes: no holes
+    z.rec_cap = 1;
+    z.slots = std::max<uint64_t>(n, 1);
+    z.hole_cap = kCenChunk;
+    return z;
+  }
   if (h->opt.algorithm == BH_ALGO_PERSISTENT) {
     // the default grid is the resident blocks, at most kMaxBlocksPerCu
     const uint64_t bpc = h->opt.blocks_per_cu > 0 ? (uint64_t)h->opt.blocks_per_cu
@@ -774,8 +844,13 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
     *out = h;
     return r;
   }
+  if (h->opt.algorithm == BH_ALGO_EVENT && h->opt.precision != BH_PRECISION_FP64) {
+    int r = fail(h, BH_ERR_ARG, "the event-split algorithm runs FP64 only");
+    *out = h;
+    return r;
+  }
   if (h->opt.precision < 0 || h->opt.precision > 1 || h->opt.algorithm < 0 ||
-      h->opt.algorithm > 1 || h->opt.tally_mode < 0 || h->opt.tally_mode > 4 ||
+      h->opt.algorithm > 2 || h->opt.tally_mode < 0 || h->opt.tally_mode > 4 ||
       !(h->opt.fp32_tol >= 0.0 && h->opt.fp32_tol < 1.0) || h->opt.general_mesh < 0 ||
       h->opt.general_mesh > 1 || h->opt.reserved != 0) {
     int r = fail(h, BH_ERR_ARG, "invalid bh_options");
@@ -803,6 +878,7 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
   HIPCHK(h, hipMalloc(&h->d_cacc, sizeof(CensusAcc)));
   HIPCHK(h, hipMalloc(&h->d_ctrs, sizeof(DevCounters)));
   HIPCHK(h, hipMalloc(&h->d_queue, sizeof(PersistentQueue)));
+  HIPCHK(h, hipMalloc(&h->d_ev_count, 4 * sizeof(unsigned long long)));
   HIPCHK(h, hipHostMalloc(&h->h_pinned, 4096, hipHostMallocDefault));
   {  // the no-allocation entry point's device memory on this device
     const hipError_t e = scratch_on_device(device, &h->scratch);
@@ -880,6 +956,14 @@ void bh_destroy(bh_handle* h) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
   }
+  release(h->ev_abs);
+  release(h->ev_track);
+  release(h->ev_segs);
+  release(h->ev_list0);
+  release(h->ev_list1);
+  release(h->ev_scatter);
+  release(h->ev_sc);
+  if (h->d_ev_count) (void)hipFree(h->d_ev_count);
   release(h->inst_first_dev);
   release(h->inst_rep);
   release(h->inst_exit);
@@ -1721,6 +1805,9 @@ int bh_transport_photons_noalloc(bh_handle* h, bh_photon* photons, uint64_t n, d
   if (!h->scratch)
     return fail(h, BH_ERR_DEVICE, "no scratch memory on device %d: %s", h->device,
                 h->scratch_err.c_str());
+  if (h->opt.algorithm == BH_ALGO_EVENT)
+    return fail(h, BH_ERR_STATE,
+                "bh_transport_photons_noalloc runs the history or persistent algorithm");
   HIPCHK(h, hipSetDevice(h->device));
   if (stats) std::memset(stats, 0, sizeof *stats);
   if (n == 0) return BH_OK;

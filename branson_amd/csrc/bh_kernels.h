@@ -406,6 +406,122 @@ __global__ __launch_bounds__(256) void k_source_fill(
   }
 }
 
+// Photon p of the step's source, from its draws u[0..6] of RNG(seed,
+// stream) at counters 0..6 (a face-source photon uses six): the sampling of
+// set_initial_census_photon / set_emission_photon /
+// set_boundary_source_photon (src/source.h:27-81) in the reference's order.
+// Shared by the source kernel and the transport kernels that make new
+// photons where they are transported.
+struct SourcePhoton {
+  double x, y, z, u, v, w, E, life;
+  uint64_t ctr, stream;
+  uint32_t cell, group;
+  uint8_t stype;
+};
+
+// The stream and source record of new photon p in cell c.
+__device__ __forceinline__ SourceInst source_record(const DevCells& cells, uint32_t c,
+                                                    const SourceInst& one,
+                                                    const SourceInst* __restrict__ inst) {
+  return inst ? inst[cells.cell_inst ? cells.cell_inst[c] : 0u] : one;
+}
+
+__device__ __forceinline__ uint64_t source_stream(const SourceInst& sp,
+                                                  const uint64_t* __restrict__ off, uint64_t p) {
+  return sp.stream_base + (p - off[sp.first_cell]);
+}
+
+__device__ __forceinline__ SourcePhoton make_source_photon(
+    const DevCells& cells, const double* __restrict__ emission_E,
+    const double* __restrict__ source_E, const int8_t* __restrict__ face_of,
+    const uint64_t* __restrict__ off, uint64_t p, uint32_t c, const SourceInst& sp, double dt,
+    int mode, const double* u) {
+  const uint64_t local = p - off[c];
+  const uint64_t n_user = sp.n_user;
+  const double total_E = sp.total_E;
+  const CellGeo& g = cells.geo[c];
+  const double nd[6] = {g.nodes[0], g.nodes[1], g.nodes[2], g.nodes[3], g.nodes[4], g.nodes[5]};
+  SourcePhoton o;
+  int k = 0;
+  const uint32_t n_em = photons_for(n_user, emission_E[c], total_E);
+  if (mode == 1 || local < n_em) {  // set_initial_census_photon / set_emission_photon
+    o.E = emission_E[c] / (double)n_em;
+    o.x = nd[0] + u[0] * (nd[1] - nd[0]);  // get_uniform_position_in_cell
+    o.y = nd[2] + u[1] * (nd[3] - nd[2]);
+    o.z = nd[4] + u[2] * (nd[5] - nd[4]);
+    const double mu = u[3] * 2.0 - 1.0;  // get_uniform_angle
+    const double phi = u[4] * 2.0 * kPi;
+    const double sin_theta = sqrt(1.0 - mu * mu);
+    o.u = sin_theta * cos(phi);
+    o.v = sin_theta * sin(phi);
+    o.w = mu;
+    k = 5;
+    if (mode == 1) {
+      o.life = kC * dt;
+      o.stype = 0;
+    } else {
+      o.life = u[k++] * kC * dt;
+      o.stype = 2;
+    }
+  } else {  // set_boundary_source_photon
+    const uint32_t n_src = photons_for(n_user, source_E[c], total_E);
+    o.E = source_E[c] / (double)n_src;
+    const int face = face_of[c];
+    // get_uniform_position_on_face (src/sampling_functions.h:32-53)
+    if (face == 0 || face == 1) {
+      o.x = (face == 0) ? nd[0] : nd[1];
+      o.y = nd[2] + u[0] * (nd[3] - nd[2]);
+      o.z = nd[4] + u[1] * (nd[5] - nd[4]);
+    } else if (face == 2 || face == 3) {
+      o.x = nd[0] + u[0] * (nd[1] - nd[0]);
+      o.y = (face == 2) ? nd[2] : nd[3];
+      o.z = nd[4] + u[1] * (nd[5] - nd[4]);
+    } else {
+      o.x = nd[0] + u[0] * (nd[1] - nd[0]);
+      o.y = nd[2] + u[1] * (nd[3] - nd[2]);
+      o.z = (face == 4) ? nd[4] : nd[5];
+    }
+    // get_source_angle_on_face (src/sampling_functions.h:94-122): inward
+    // sign on x faces only, as the reference does
+    const double theta = acos(sqrt(u[2]));
+    const double phi = u[3] * 2.0 * kPi;
+    const double sign = (face % 2) ? -1.0 : 1.0;
+    const double ct = cos(theta), st = sin(theta);
+    if (face == 0 || face == 1) {
+      o.u = ct * sign;
+      o.v = st * sin(phi);
+      o.w = st * cos(phi);
+    } else if (face == 2 || face == 3) {
+      o.u = st * sin(phi);
+      o.v = ct;
+      o.w = st * cos(phi);
+    } else {
+      o.u = st * cos(phi);
+      o.v = st * sin(phi);
+      o.w = ct;
+    }
+    k = 4;
+    o.life = u[k++] * kC * dt;
+    o.stype = 1;
+  }
+  o.group = (uint32_t)floor(u[k++] * (double)cells.G);
+  o.ctr = (uint64_t)k;
+  o.stream = source_stream(sp, off, p);
+  o.cell = c;
+  return o;
+}
+
+constexpr int kSourceDraws = 7;  // the most draws a source photon takes
+
+// Draws photon p in cell c takes: 7 for emission, 6 for a face-source or an
+// initial-census photon.
+__device__ __forceinline__ int source_draw_count(const double* __restrict__ emission_E,
+                                                 const uint64_t* __restrict__ off, uint64_t p,
+                                                 uint32_t c, const SourceInst& sp, int mode) {
+  if (mode == 1) return 6;
+  return (p - off[c] < photons_for(sp.n_user, emission_E[c], sp.total_E)) ? 7 : 6;
+}
+
 __device__ __forceinline__ void source_one(const DevCells& cells,
                                            const double* __restrict__ emission_E,
                                            const double* __restrict__ source_E,
@@ -414,58 +530,33 @@ __device__ __forceinline__ void source_one(const DevCells& cells,
                                            uint32_t c, const SourceInst& one,
                                            const SourceInst* __restrict__ inst, double dt,
                                            int mode, const PhotonSoA& W, uint64_t out_base) {
-  const uint64_t local = p - off[c];
-  const SourceInst sp = inst ? inst[cells.cell_inst ? cells.cell_inst[c] : 0u] : one;
-  const uint64_t n_user = sp.n_user;
-  const double total_E = sp.total_E;
-  const CellGeo& g = cells.geo[c];
-  const double nd[6] = {g.nodes[0], g.nodes[1], g.nodes[2], g.nodes[3], g.nodes[4], g.nodes[5]};
-
-  Rng rng{0ull, sp.stream_base + (p - off[sp.first_cell]), sp.ctr_hi};
-  double x, y, z, u, v, w, E, life;
-  uint8_t stype;
-  const uint32_t n_em = photons_for(n_user, emission_E[c], total_E);
-  if (mode == 1) {  // set_initial_census_photon (src/source.h:65-81)
-    E = emission_E[c] / (double)n_em;
-    pos_in_cell(nd, rng, x, y, z);
-    uniform_angle(rng, u, v, w);
-    life = kC * dt;
-    stype = 0;
-  } else if (local < n_em) {  // set_emission_photon (src/source.h:27-43)
-    E = emission_E[c] / (double)n_em;
-    pos_in_cell(nd, rng, x, y, z);
-    uniform_angle(rng, u, v, w);
-    life = rng.next() * kC * dt;
-    stype = 2;
-  } else {  // set_boundary_source_photon (src/source.h:46-62)
-    const uint32_t n_src = photons_for(n_user, source_E[c], total_E);
-    E = source_E[c] / (double)n_src;
-    const int face = face_of[c];
-    pos_on_face(nd, face, rng, x, y, z);
-    source_angle(face, rng, u, v, w);
-    life = rng.next() * kC * dt;
-    stype = 1;
-  }
-  const uint32_t group = (uint32_t)floor(rng.next() * (double)cells.G);
-  const uint64_t o = out_base + p;
+  const SourceInst sp = source_record(cells, c, one, inst);
+  double u[kSourceDraws];
+  Rng rng{0ull, source_stream(sp, off, p), sp.ctr_hi};
+  const int nd = source_draw_count(emission_E, off, p, c, sp, mode);
+#pragma unroll
+  for (int k = 0; k < kSourceDraws; ++k) u[k] = (k < nd) ? rng.next() : 0.0;
+  const SourcePhoton o = make_source_photon(cells, emission_E, source_E, face_of, off, p, c, sp,
+                                            dt, mode, u);
+  const uint64_t i = out_base + p;
   // an initial census (mode 1) is in the reference's order by p; new photons
   // carry p too
-  W.key[o] = (uint32_t)p;
-  W.x[o] = x;
-  W.y[o] = y;
-  W.z[o] = z;
-  W.u[o] = u;
-  W.v[o] = v;
-  W.w[o] = w;
-  W.E[o] = E;
-  W.E0[o] = E;
-  W.life[o] = life;
-  W.ctr[o] = rng.ctr;
-  W.stream[o] = rng.stream;
-  W.cell[o] = c;
-  W.group[o] = group;
-  W.desc[o] = kPass;
-  W.stype[o] = stype;
+  W.key[i] = (uint32_t)p;
+  W.x[i] = o.x;
+  W.y[i] = o.y;
+  W.z[i] = o.z;
+  W.u[i] = o.u;
+  W.v[i] = o.v;
+  W.w[i] = o.w;
+  W.E[i] = o.E;
+  W.E0[i] = o.E;
+  W.life[i] = o.life;
+  W.ctr[i] = o.ctr;
+  W.stream[i] = o.stream;
+  W.cell[i] = o.cell;
+  W.group[i] = o.group;
+  W.desc[i] = kPass;
+  W.stype[i] = o.stype;
 }
 
 // ---------------------------------------------------------------------------

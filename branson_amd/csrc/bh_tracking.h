@@ -431,6 +431,70 @@ struct SegInfo {
   double r_sa, r_f, r_sig;  // the opacities that test used
 };
 
+// The draws of an FP64 segment taken lane by lane, around ONE Threefry site
+// (a draw per loop trip keeps the replay, adjoint and event kernels small):
+// after an effective scatter (L.pending) the new direction, the re-emission
+// test and, when it re-samples, the group (src/history_based_transport.h:
+// 95-100); then, with `distance`, the distance to scatter (:62-63), which is
+// returned (1e100 when there is none).  sg / total_sigma_s follow a group
+// change.
+template <bool kInfo, bool kLat>
+__device__ __forceinline__ double lane_draws(const DevCells& cells, const CellView<kLat>& cv,
+                                             double f, Live<double>& L, double2& sg,
+                                             double& total_sigma_s, SegInfo* si, bool distance) {
+  const uint32_t G = cells.G;
+  double d_s = 1.0e100;
+  int st = L.pending ? kDrawMu : (distance ? kDrawDistance : kDrawDone);
+  if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
+  double mu = 0.0;
+  while (st != kDrawDone) {
+    const double r = L.rng.next();  // the loop's only Threefry site
+    if (st == kDrawMu) {  // get_uniform_angle (src/sampling_functions.h:56-71)
+      mu = r * 2.0 - 1.0;
+      st = kDrawPhi;
+    } else if (st == kDrawPhi) {
+      const double phi = r * 2.0 * kPi;
+      const double sin_theta = sqrt(1.0 - mu * mu);
+      double sp, cp;
+      sincos(phi, &sp, &cp);
+      L.u = sin_theta * cp;
+      L.v = sin_theta * sp;
+      L.w = mu;
+      st = kDrawTest;
+    } else if (st == kDrawTest) {  // re-emission test (:97)
+      const bool resample = r > (sg.y / total_sigma_s);
+      if constexpr (kInfo) {
+        si->reemit = resample ? 2 : 1;
+        si->r_sa = sg.x;
+        si->r_f = f;
+        si->r_sig = total_sigma_s;
+      }
+      st = resample ? kDrawGroup
+                    : ((distance && total_sigma_s > 0.0) ? kDrawDistance : kDrawDone);
+    } else if (st == kDrawGroup) {  // sample_emission_group walk (:98)
+      double cdf = r;
+      int g = -1;
+      const double norm = 1.0 / (cv.sigma(0).x * (double)G);
+      while (cdf > 0) {
+        g++;
+        if ((uint32_t)g >= G) {
+          g = (int)G - 1;
+          break;
+        }
+        cdf -= cv.sigma((uint32_t)g).x * norm;
+      }
+      L.group = (uint32_t)g;
+      sg = cv.sigma((uint32_t)g);
+      total_sigma_s = (1.0 - f) * sg.x + sg.y;
+      st = (distance && total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
+    } else {  // distance to scatter (:62-63)
+      d_s = -log(r) / total_sigma_s;
+      st = kDrawDone;
+    }
+  }
+  return d_s;
+}
+
 // ---- FP64: the reference's arithmetic, operation for operation ----------
 // pre: the cell's view loaded before the wave's draws, so that the load's
 // latency hides behind them (the draws' LDS fences keep the compiler from
@@ -504,53 +568,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
     }
     if (total_sigma_s > 0.0) d_s = -log(draw()) / total_sigma_s;  // distance to scatter (:62-63)
   } else {  // lane by lane: one Threefry site, a draw per loop trip
-    int st = L.pending ? kDrawMu : kDrawDistance;
-    if (st == kDrawDistance && !(total_sigma_s > 0.0)) st = kDrawDone;
-    double mu = 0.0;
-    while (st != kDrawDone) {
-      const double r = L.rng.next();  // the loop's only Threefry site
-      if (st == kDrawMu) {  // get_uniform_angle (src/sampling_functions.h:56-71)
-        mu = r * 2.0 - 1.0;
-        st = kDrawPhi;
-      } else if (st == kDrawPhi) {
-        const double phi = r * 2.0 * kPi;
-        const double sin_theta = sqrt(1.0 - mu * mu);
-        double sp, cp;
-        sincos(phi, &sp, &cp);
-        L.u = sin_theta * cp;
-        L.v = sin_theta * sp;
-        L.w = mu;
-        st = kDrawTest;
-      } else if (st == kDrawTest) {  // re-emission test (:97)
-        const bool resample = r > (sg.y / total_sigma_s);
-        if constexpr (kInfo) {
-          si->reemit = resample ? 2 : 1;
-          si->r_sa = sg.x;
-          si->r_f = f;
-          si->r_sig = total_sigma_s;
-        }
-        st = resample ? kDrawGroup : kDrawDistance;
-      } else if (st == kDrawGroup) {  // sample_emission_group walk (:98)
-        double cdf = r;
-        int g = -1;
-        const double norm = 1.0 / (cv.sigma(0).x * (double)G);
-        while (cdf > 0) {
-          g++;
-          if ((uint32_t)g >= G) {
-            g = (int)G - 1;
-            break;
-          }
-          cdf -= cv.sigma((uint32_t)g).x * norm;
-        }
-        L.group = (uint32_t)g;
-        sg = cv.sigma((uint32_t)g);
-        total_sigma_s = (1.0 - f) * sg.x + sg.y;
-        st = (total_sigma_s > 0.0) ? kDrawDistance : kDrawDone;
-      } else {  // distance to scatter (:62-63)
-        d_s = -log(r) / total_sigma_s;
-        st = kDrawDone;
-      }
-    }
+    d_s = lane_draws<kInfo, kLat>(cells, cv, f, L, sg, total_sigma_s, si, true);
   }
   L.pending = false;
   const double sigma_a = sg.x;
@@ -1024,8 +1042,10 @@ struct CensusLocal {
 // (not handed to the FP64 replay) this iteration, from physical slot i of W.
 // Census photons go to the next slots of the wave's chunk of C (a new chunk,
 // one atomic, when it fills), with the reference's census life, and their
-// transport index is marked for the census order.
-template <typename R, bool kLat>
+// transport index is marked for the census order.  kDense: the wave takes
+// exactly the slots its census photons fill, one atomic per call (kernels
+// launched many times per step, which must leave no holes).
+template <typename R, bool kLat, bool kDense = false>
 __device__ __forceinline__ void census_append(bool ended, const DevCells& cells, uint64_t i,
                                               const Live<R>& L, const CensusRun& cr,
                                               CensusLocal& loc) {
@@ -1047,20 +1067,21 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
   const CensusOut co = arg_census();
   const PhotonSoA W = arg_stores();
   const uint32_t cnt = (uint32_t)__popcll(m);
-  const uint32_t fill = loc.cfill;
-  const bool take = fill + cnt > kCenChunk;  // wave-uniform
+  const uint32_t fill = kDense ? 0u : loc.cfill;
+  const bool take = kDense || fill + cnt > kCenChunk;  // wave-uniform
   uint64_t fresh = 0;
   if (take) {
     const int leader = __ffsll((long long)m) - 1;
     if ((int)(threadIdx.x & (kWave - 1)) == leader)
-      fresh = atomicAdd(&co.acc->n_reserved, (unsigned long long)kCenChunk);
+      fresh = atomicAdd(&co.acc->n_reserved, (unsigned long long)(kDense ? cnt : kCenChunk));
     fresh = shfl_u64(fresh, leader);
     fresh = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(fresh >> 32)) << 32) |
             __builtin_amdgcn_readfirstlane((uint32_t)fresh);  // wave-uniform
   }
   if (cen) {
     const uint32_t q = fill + lane_rank(m);
-    const uint64_t s = q < kCenChunk ? loc.cbase + q : fresh + (q - kCenChunk);
+    const uint64_t s = kDense ? fresh + q
+                              : (q < kCenChunk ? loc.cbase + q : fresh + (q - kCenChunk));
     const uint64_t t = (i >= cr.n_cen) ? i - cr.n_cen : cr.n_new + W.key[i];
     if (s < co.slots) {
       store_live<R, kLat>(cells, co.C, s, L);
@@ -1079,11 +1100,13 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
     }
   }
   loc.n_cen += cnt;
-  if (take) {
-    loc.cbase = fresh;
-    loc.cfill = fill + cnt - kCenChunk;
-  } else {
-    loc.cfill = fill + cnt;
+  if constexpr (!kDense) {
+    if (take) {
+      loc.cbase = fresh;
+      loc.cfill = fill + cnt - kCenChunk;
+    } else {
+      loc.cfill = fill + cnt;
+    }
   }
 }
 
@@ -1243,6 +1266,162 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
   tally.finish(cells.n_cells);
   add_wave_counters(ctrs, segs, flushes, stuck);
   if (co.on) census_flush(loc);
+}
+
+// ---- Event-split form -------------------------------------------------------
+// The reference's event-based algorithm (src/event_based_transport.h:
+// 262-305: precompute, distances, state update, per-event processing, then
+// compaction of the active list) on the device, with the history form's
+// exact per-photon arithmetic and draw order, so both give the same events:
+//   k_ev_flight   one segment of every active photon: opacities, its
+//                 distance draw, distances to scatter / face / census,
+//                 implicit-capture deposit, move, then the event -- a face
+//                 crossing or reflection, census, exit and kill are handled
+//                 here; an effective scatter is queued;
+//   k_ev_scatter  every queued scatter at full width: new direction, the
+//                 re-emission test and the group walk (4-5 draws);
+// and each pass writes the photons still alive into the next active list
+// with one atomic per wave (ballot, then each lane's rank among the set
+// lanes): the live photons stay packed, so every wave of the next pass is
+// full.  Between passes a photon's open cell visit (its accumulated
+// deposits), segment count and last face live in EventState.
+struct EventState {
+  double* t_abs;
+  double* t_track;
+  uint32_t* segs;
+  uint8_t* sc;
+  uint32_t* list[2];           // active photon slots (ping-pong)
+  uint32_t* scatter;           // slots whose flight ended in an effective scatter
+  unsigned long long* count;   // [0], [1]: active counts; [2]: scatter count
+};
+
+// Wave-aggregated append of the lanes with `want` set: one atomic per wave.
+// Every lane of the wave at the same program point.
+__device__ __forceinline__ void list_append(bool want, uint32_t v, uint32_t* __restrict__ list,
+                                            unsigned long long* n) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(n, (unsigned long long)__popcll(m));
+  base = shfl_u64(base, leader);
+  if (want) list[base + lane_rank(m)] = v;
+}
+
+// Active list = every photon; open visits empty.
+__global__ void k_ev_init(EventState ev, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    ev.count[0] = n;
+    ev.count[1] = 0;
+    ev.count[2] = 0;
+  }
+  if (i >= n) return;
+  ev.list[0][i] = (uint32_t)i;
+  ev.t_abs[i] = 0.0;
+  ev.t_track[i] = 0.0;
+  ev.segs[i] = 0;
+  ev.sc[i] = 0;
+}
+
+// Before a pass pair: the lists it fills start empty.
+__global__ void k_ev_clear(EventState ev, int nxt) {
+  ev.count[nxt] = 0;
+  ev.count[2] = 0;
+}
+
+template <class Tally, bool kLat>
+BH_TRACK_KERNEL void k_ev_flight(const PhotonSoA W, const CensusOut co, DevCells cells,
+                                 EventState ev, int cur, uint64_t ctr_hi, Tally tally,
+                                 DevCounters* ctrs) {
+  if (!args_ok(W, co, ctrs)) return;
+  const CensusRun cr = census_run(co, nullptr);
+  CensusLocal loc;
+  bind_lds(tally, cells.n_cells);
+  tally.init(cells.n_cells);
+  const uint64_t n = ev.count[cur];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long flushes = 0, segs = 0, stuck = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint64_t j = base + threadIdx.x;
+    const bool active = j < n;
+    const uint32_t i = active ? ev.list[cur][j] : 0u;
+    Live<double> L;
+    L.rng = Rng{0, 0, 0};
+    L.pending = false;
+    if (active) {
+      load_live<double, kLat>(cells, arg_stores(), i, ctr_hi, L);
+      L.t_abs = ev.t_abs[i];
+      L.t_track = ev.t_track[i];
+      L.segs = ev.segs[i];
+      L.sc = ev.sc[i];
+    }
+    Flush fo;
+    fo.has = false;
+    fo.kill = 0.0;
+    bool ended = false;
+    const CellView<kLat> cv(cells, active ? L.cell : 0u, L.ijk);
+    if (active && segment<Tally::kTrack, kLat>(cells, L, fo, nullptr, nullptr, &cv)) {
+      ended = true;
+      if (!co.on) store_live<double, kLat>(cells, arg_stores(), i, L);
+    }
+    const bool alive = active && !ended;
+    if (alive) {
+      store_live<double, kLat>(cells, arg_stores(), i, L);
+      ev.t_abs[i] = L.t_abs;
+      ev.t_track[i] = L.t_track;
+      ev.segs[i] = L.segs;
+      ev.sc[i] = (uint8_t)L.sc;
+    }
+    segs += (unsigned long long)wave_sum_u64(ended ? L.segs : 0u);
+    stuck += (unsigned long long)__popcll(__ballot(ended && L.desc == kStuck));
+    list_append(alive && L.pending, i, ev.scatter, &ev.count[2]);
+    list_append(alive && !L.pending, i, ev.list[cur ^ 1], &ev.count[cur ^ 1]);
+    if (co.on) census_append<double, kLat, true>(ended, cells, i, L, cr, loc);
+    flushes += (unsigned long long)__popcll(__ballot(fo.has));
+    tally.push(fo);
+  }
+  tally.finish(cells.n_cells);
+  if ((threadIdx.x & (kWave - 1)) == 0) {  // segs already wave sums
+    atomicAdd(&ctrs->segments, segs);
+    atomicAdd(&ctrs->flushes, flushes);
+    if (stuck) atomicAdd(&ctrs->stuck, stuck);
+  }
+  if (co.on) census_flush(loc);
+}
+
+template <bool kLat>
+__global__ __launch_bounds__(256) void k_ev_scatter(PhotonSoA W, DevCells cells, EventState ev,
+                                                    int cur, uint64_t ctr_hi) {
+  const uint64_t n = ev.count[2];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint64_t j = base + threadIdx.x;
+    const bool active = j < n;
+    uint32_t i = 0;
+    if (active) {
+      i = ev.scatter[j];
+      Live<double> L;
+      L.cell = W.cell[i];
+      L.group = W.group[i];
+      if constexpr (kLat) lattice_ijk(cells, L.cell, L.ijk);
+      L.rng.ctr = W.ctr[i];
+      L.rng.stream = W.stream[i];
+      L.rng.ctr_hi = photon_ctr_hi(cells, L.cell, ctr_hi);
+      L.pending = true;
+      const CellView<kLat> cv(cells, L.cell, L.ijk);
+      double2 sg = cv.sigma(L.group);
+      double total_sigma_s = (1.0 - cv.f) * sg.x + sg.y;
+      (void)lane_draws<false, kLat>(cells, cv, cv.f, L, sg, total_sigma_s, nullptr, false);
+      W.u[i] = L.u;
+      W.v[i] = L.v;
+      W.w[i] = L.w;
+      W.group[i] = L.group;
+      W.ctr[i] = L.rng.ctr;
+    }
+    list_append(active, i, ev.list[cur ^ 1], &ev.count[cur ^ 1]);
+  }
 }
 
 // FP64 replay of the histories the FP32 form stopped: each starts again

@@ -53,8 +53,11 @@ enum {
 };
 /* loop organisation */
 enum {
-  BH_ALGO_HISTORY = 0,   /* one work-item per photon, loop to completion  */
-  BH_ALGO_PERSISTENT = 1 /* persistent waves refill finished lanes        */
+  BH_ALGO_HISTORY = 0,    /* one work-item per photon, loop to completion  */
+  BH_ALGO_PERSISTENT = 1, /* persistent waves refill finished lanes        */
+  BH_ALGO_EVENT = 2       /* event-split passes over a compacted list of live
+                             photons (src/event_based_transport.h:262-305);
+                             FP64 only, same events as the other forms     */
 };
 
 typedef struct bh_options {

@@ -137,7 +137,9 @@ def _transport_both(d, n_user, engine_kw, n_threads=8):
                                         (capi.ALGO_HISTORY, 3), (capi.ALGO_HISTORY, 4),
                                         (capi.ALGO_PERSISTENT, 0), (capi.ALGO_PERSISTENT, 1),
                                         (capi.ALGO_PERSISTENT, 2), (capi.ALGO_PERSISTENT, 3),
-                                        (capi.ALGO_PERSISTENT, 4)])
+                                        (capi.ALGO_PERSISTENT, 4), (capi.ALGO_EVENT, 0),
+                                        (capi.ALGO_EVENT, 2), (capi.ALGO_EVENT, 3),
+                                        (capi.ALGO_EVENT, 4)])
 @pytest.mark.parametrize("deck,n_user", [("cube_40.xml", 300_000),
                                          ("marshak_slab_100.xml", 50_000)])
 def test_transport_photons_fp64_matches_oracle(deck, n_user, algo, tally):
@@ -194,7 +196,7 @@ def _worst_rel(g, o):
     return (float(r[k]) if r.size else 0.0), int(np.flatnonzero(m)[k]) if r.size else -1
 
 
-@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT])
+@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT, capi.ALGO_EVENT])
 def test_marshak_ten_steps_fp64_state_injected(algo):
     """Per step, identical inputs: census count, cells and RNG positions
     exact; per-cell absorbed/track energy and census/exit energy 1e-12."""
@@ -384,7 +386,7 @@ def test_census_written_by_the_kernels_is_deterministic(deck, n_user):
     waves finish; the census export (the reference's order, by transport
     index) and the census energy (summed over the energies stored by
     transport index) do not depend on it: two persistent runs and the
-    one-work-item-per-history form agree bit for bit, and the energy equals
+    one-work-item-per-history and event-split forms agree bit for bit, and the energy equals
     the ordered sum of the exported census within the tree-sum bound.  The
     Marshak deck starts with an initial census (step 1)."""
     d = _deck(deck)
@@ -392,7 +394,8 @@ def test_census_written_by_the_kernels_is_deterministic(deck, n_user):
     cells, ph, clk = _step_inputs(sim)
     census_E = ph["census_E"] if clk["step"] == 1 else None
     runs = []
-    for algo in (capi.ALGO_PERSISTENT, capi.ALGO_PERSISTENT, capi.ALGO_HISTORY):
+    for algo in (capi.ALGO_PERSISTENT, capi.ALGO_PERSISTENT, capi.ALGO_HISTORY,
+                 capi.ALGO_EVENT):
         with _engine(algorithm=algo) as e:
             e.set_mesh_cells(cells)
             e.source(ph["emission_E"], ph["source_E"], census_E, ph["source_face"], n_user,
@@ -411,7 +414,7 @@ def test_census_written_by_the_kernels_is_deterministic(deck, n_user):
     assert abs(E0 - serial) <= 2.0 * (n0 + 64) * 2.0 ** -53 * serial
 
 
-@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT])
+@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT, capi.ALGO_EVENT])
 def test_group_dependent_opacities(algo):
     """Per-group sigma_a (allowed by the C-ABI although Branson's decks fill
     every group alike): the re-emission group walk sees a non-uniform CDF
