@@ -22,6 +22,7 @@ REFLECT, VACUUM, ELEMENT, SOURCE, PROCESSOR = range(5)
 EXIT, PASS, CENSUS, SCATTER, KILLED, BOUND = range(6)
 PRECISION_FP64, PRECISION_FP32_TRACK = 0, 1
 ALGO_HISTORY, ALGO_PERSISTENT, ALGO_EVENT = 0, 1, 2
+SOURCE_AUTO, SOURCE_IN_TRANSPORT, SOURCE_KERNEL = 0, 1, 2
 
 # the reference's 120-byte AOS Photon (src/photon.h:172-181)
 PHOTON_DTYPE = np.dtype([
@@ -56,7 +57,8 @@ class Options(C.Structure):
                 ("blocks_per_cu", C.c_int32), ("tally_mode", C.c_int32),
                 ("refill_threshold", C.c_int32), ("reserve_photons", C.c_uint64),
                 ("fp32_tol", C.c_double), ("batch_photons", C.c_int32),
-                ("step_graph", C.c_int32), ("general_mesh", C.c_int32), ("reserved", C.c_int32)]
+                ("step_graph", C.c_int32), ("general_mesh", C.c_int32),
+                ("source_mode", C.c_int32)]
 
 
 class Cells(C.Structure):
@@ -171,12 +173,13 @@ class Engine:
     def __init__(self, device: int = 0, precision: int = PRECISION_FP64,
                  algorithm: int = ALGO_HISTORY, tally_mode: int = 0, blocks_per_cu: int = 0,
                  refill_threshold: int = 0, reserve_photons: int = 0, fp32_tol: float = 0.0,
-                 block_size: int = 0, batch_photons: int = 0, general_mesh: bool = False):
+                 block_size: int = 0, batch_photons: int = 0, general_mesh: bool = False,
+                 source_mode: int = 0):
         L = lib()
         self._h = C.c_void_p()
         self.opts = Options(precision, algorithm, block_size, blocks_per_cu, tally_mode,
                             refill_threshold, reserve_photons, fp32_tol, batch_photons, 0,
-                            int(general_mesh), 0)
+                            int(general_mesh), source_mode)
         rc = L.bh_create(device, C.byref(self.opts), C.byref(self._h))
         if rc != BH_OK:
             msg = L.bh_last_error(self._h).decode() if self._h else "bh_create failed"
@@ -513,12 +516,14 @@ class Driver:
                  tally_mode: int = 0, blocks_per_cu: int = 0, refill_threshold: int = 0,
                  reserve_photons: int = 0, fp32_tol: float = 0.0, block_size: int = 0,
                  batch_photons: int = 0, step_graph: bool = False,
-                 general_mesh: bool = False, device_physics: bool = False):
+                 general_mesh: bool = False, device_physics: bool = False,
+                 source_mode: int = 0):
         L = _host_lib()
         self._d = C.c_void_p()
         cfg = DriverConfig(Options(precision, algorithm, block_size, blocks_per_cu, tally_mode,
                                    refill_threshold, reserve_photons, fp32_tol, batch_photons,
-                                   int(step_graph), int(general_mesh), 0), device, n_groups,
+                                   int(step_graph), int(general_mesh), source_mode), device,
+                           n_groups,
                            rank, n_ranks, n_photons, t_stop, int(device_physics), 0)
         rc = L.bh_driver_create(os.fsencode(xml_path), C.byref(cfg), C.byref(self._d))
         if rc != BH_OK:

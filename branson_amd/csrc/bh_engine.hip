@@ -162,6 +162,13 @@ struct bh_handle {
   DArr<double> em, src, cen;
   DArr<int8_t> face;
   bool has_energy = false, has_census_E = false;
+  // the step's new photons: exclusive prefix of the per-cell counts and the
+  // cell hints; with the persistent algorithm they are made inside the
+  // transport kernel (src_deferred) rather than written to W
+  DArr<uint64_t> soff;
+  DArr<uint32_t> src_hint;
+  SourceDesc sd{};
+  bool src_deferred = false;
 
   // photons: W = [census | new], C = next census / scratch
   PhotonStore W, C;
@@ -536,11 +543,12 @@ bool use_lds_tally(const bh_handle* h, bool persistent) {
 // device, in h->d_ctrs->replays); same tally sink as the kernel before it.
 template <class T>
 int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const ReplayList& rl,
-                  T t, size_t shm, const CensusOut& co, const StepParams* P, const Sinks& s) {
+                  T t, size_t shm, const CensusOut& co, const StepParams* P, const Sinks& s,
+                  const SourceDesc& sd) {
   // the replay's share of the transport time (not inside a captured graph)
   h->replay_timed = P == nullptr;
   if (h->replay_timed) HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
-  k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(W, co, cells, rl,
+  k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(W, co, sd, cells, rl,
                                                                      s.ctr_hi, t, s.ctrs, P);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
@@ -607,8 +615,10 @@ constexpr int kMaxBlocksPerCu = 8;
 // With P (a step being captured as a graph) the persistent kernel reads n on
 // the device and n only sizes the grid and buffers.  co: census output
 // (co.on = 0: photons stay in place in W).
+// sd.on: the photons from the census count on are made inside the
+// persistent kernel (and again in the FP64 replay); no other form makes them.
 int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepParams* P,
-                     const CensusOut& co, const Sinks& s) {
+                     const CensusOut& co, const Sinks& s, const SourceDesc& sd = SourceDesc{}) {
   if (n == 0) return BH_OK;
   const DevCells cells = dev_cells(h);
   const int block = h->opt.block_size;
@@ -628,6 +638,8 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
     TRY(ensure(h, h->rep_E, cap));
     rl = ReplayList{h->rep_idx.p, h->rep_first.p, h->rep_E.p};
   }
+  if (sd.on && !persistent)
+    return fail(h, BH_ERR_STATE, "only the persistent kernel makes new photons itself");
   if (h->opt.algorithm == BH_ALGO_EVENT) {
     if (P) return fail(h, BH_ERR_STATE, "the event-split algorithm is not captured as a graph");
     return launch_event(h, W, n, co, s, lds);
@@ -643,7 +655,7 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
           k_transport_history<T, float, false><<<grid_for(n, block), block, shm, h->stream>>>(
               W, co, cells, n, ctr_hi, t, ctrs, rl);
         HIPCHK(h, hipGetLastError());
-        return launch_replay(h, cells, W, rl, t, shm, co, P, s);
+        return launch_replay(h, cells, W, rl, t, shm, co, P, s, sd);
       }
       if (lat)
         k_transport_history<T, double, true><<<grid_for(n, block), block, shm, h->stream>>>(
@@ -677,28 +689,24 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
                                       sizeof(PersistentQueue) / 8);
   return with_tally(h, lds, s, [&](auto t, size_t shm) -> int {
     using T = decltype(t);
+    // one instantiation per (precision, records, in-kernel photon making)
+    auto go = [&](auto* k) {
+      k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
+          W, co, sd, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
+    };
+    const bool gen = sd.on != 0;
     if (fp32) {
-      if (lat) {
-        auto* k = &k_transport_persistent<T, float, true>;
-        k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-            W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
-      } else {
-        auto* k = &k_transport_persistent<T, float, false>;
-        k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-            W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
-      }
+      if (lat && gen) go(&k_transport_persistent<T, float, true, true>);
+      else if (lat) go(&k_transport_persistent<T, float, true, false>);
+      else if (gen) go(&k_transport_persistent<T, float, false, true>);
+      else go(&k_transport_persistent<T, float, false, false>);
       HIPCHK(h, hipGetLastError());
-      return launch_replay(h, cells, W, rl, t, shm, co, P, s);
+      return launch_replay(h, cells, W, rl, t, shm, co, P, s, sd);
     }
-    if (lat) {
-      auto* k = &k_transport_persistent<T, double, true>;
-      k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-          W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
-    } else {
-      auto* k = &k_transport_persistent<T, double, false>;
-      k<<<grid_of(reinterpret_cast<const void*>(k), shm), block, shm, h->stream>>>(
-          W, co, cells, n, ctr_hi, t, ctrs, s.queue, refill, chunk, rl, P);
-    }
+    if (lat && gen) go(&k_transport_persistent<T, double, true, true>);
+    else if (lat) go(&k_transport_persistent<T, double, true, false>);
+    else if (gen) go(&k_transport_persistent<T, double, false, true>);
+    else go(&k_transport_persistent<T, double, false, false>);
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   });
@@ -810,6 +818,35 @@ int census_finish(bh_handle* h, uint64_t cap, const uint64_t* n_ptr) {
   return BH_OK;
 }
 
+// Writes the step's new photons to W behind the census (src_deferred: they
+// were left to be made inside the transport kernel).  For everything that
+// reads them from W: the history and event-split forms and the exports.
+// Whether the step's new photons are made inside the persistent transport
+// kernel (bh_options.source_mode 1).  Auto keeps the source kernel: making
+// them in the transport kernel saves their 97-byte HBM round trip but costs
+// that kernel 30-34 spilled VGPRs at its three waves per SIMD, and measured
+// slower on every workload (hohlraum 8.05 against 6.72 ms per step, cube
+// 10.2 against 9.7 ms).
+bool source_in_transport(const bh_handle* h, uint64_t) {
+  return h->opt.algorithm == BH_ALGO_PERSISTENT && h->opt.source_mode == 1;
+}
+
+int materialize_source(bh_handle* h, uint64_t n_new) {
+  if (!h->src_deferred) return BH_OK;
+  if (n_new == 0) {
+    h->src_deferred = false;
+    return BH_OK;
+  }
+  TRY(grow_store(h, h->W, h->n_cen + n_new, h->n_cen));
+  const SourceDesc& d = h->sd;
+  k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
+      dev_cells(h), d.em, d.src, d.face, d.off, n_new, d.one, d.inst, d.dt, 0, h->W.soa,
+      h->n_cen, nullptr);
+  HIPCHK(h, hipGetLastError());
+  h->src_deferred = false;
+  return BH_OK;
+}
+
 float elapsed_ms(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
@@ -852,7 +889,7 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
   if (h->opt.precision < 0 || h->opt.precision > 1 || h->opt.algorithm < 0 ||
       h->opt.algorithm > 2 || h->opt.tally_mode < 0 || h->opt.tally_mode > 4 ||
       !(h->opt.fp32_tol >= 0.0 && h->opt.fp32_tol < 1.0) || h->opt.general_mesh < 0 ||
-      h->opt.general_mesh > 1 || h->opt.reserved != 0) {
+      h->opt.general_mesh > 1 || h->opt.source_mode < 0 || h->opt.source_mode > 2) {
     int r = fail(h, BH_ERR_ARG, "invalid bh_options");
     *out = h;
     return r;
@@ -956,6 +993,8 @@ void bh_destroy(bh_handle* h) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
   }
+  release(h->soff);
+  release(h->src_hint);
   release(h->ev_abs);
   release(h->ev_track);
   release(h->ev_segs);
@@ -993,6 +1032,7 @@ int bh_sync(bh_handle* h) {
 int bh_set_mesh(bh_handle* h, const bh_cells* c) {
   if (!h || !h->stream) return BH_ERR_ARG;
   TRY(check_cells(h, c));
+  if (h->has_mesh) TRY(materialize_source(h, h->n_new));  // they read the old cells
   const uint32_t n = c->n_cells, G = c->n_groups;
   std::vector<CellGeo> geo(n);
   for (uint32_t i = 0; i < n; ++i) {
@@ -1067,6 +1107,8 @@ int bh_set_source_energy(bh_handle* h, const double* emission_E, const double* s
                          const double* census_E, const int8_t* source_face) {
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_source_energy before bh_set_mesh");
+  // photons of an earlier bh_source still to be made read these energies
+  TRY(materialize_source(h, h->n_new));
   if (!emission_E || !source_E || !source_face)
     return fail(h, BH_ERR_ARG, "emission_E, source_E and source_face are required");
   const uint32_t n = h->n_cells;
@@ -1117,7 +1159,7 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
   const uint32_t n = h->n_cells;
   const DevCells cells = dev_cells(h);
   TRY(ensure(h, h->counts, n));
-  TRY(ensure(h, h->off, n));
+  TRY(ensure(h, h->soff, n));
   const uint32_t* cinst = h->n_inst ? h->cell_inst.p : nullptr;
   HIPCHK(h, hipEventRecord(h->ev[4], h->stream));
   if (h->has_census_E) {  // step 1: initial census replaces the resident one
@@ -1128,13 +1170,13 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
                                                              census_part);
       HIPCHK(h, hipGetLastError());
     }
-    TRY(scan_u64(h, h->counts.p, n, h->off.p, 0));
+    TRY(scan_u64(h, h->counts.p, n, h->soff.p, 0));
     uint64_t n_c = 0;
     TRY(read_total(h, 0, &n_c));
     TRY(grow_store(h, h->W, n_c, 0));
     if (n_c)
       k_source_fill<<<grid_for(n_c, 256), 256, 0, h->stream>>>(
-          cells, h->cen.p, nullptr, h->face.p, h->off.p, n_c, census, icen, dt, 1, h->W.soa, 0,
+          cells, h->cen.p, nullptr, h->face.p, h->soff.p, n_c, census, icen, dt, 1, h->W.soa, 0,
           nullptr);
     HIPCHK(h, hipGetLastError());
     h->n_cen = n_c;
@@ -1142,15 +1184,20 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
   }
   k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, n, fresh, inew,
                                                             cinst, 0, h->counts.p);
-  TRY(scan_u64(h, h->counts.p, n, h->off.p, 1));
+  TRY(scan_u64(h, h->counts.p, n, h->soff.p, 1));
   uint64_t n_new = 0;
   TRY(read_total(h, 1, &n_new));
-  TRY(grow_store(h, h->W, h->n_cen + n_new, h->n_cen));
+  // the new photons: made inside the persistent transport kernel, or
+  // written to W behind the census here
+  TRY(ensure(h, h->src_hint, div_up(std::max<uint64_t>(n_new, 1), kHintStride) + 1));
+  h->sd = SourceDesc{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, inew, fresh, dt,
+                     n_new, 1, 0};
+  h->src_deferred = n_new > 0;
   if (n_new)
-    k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
-        cells, h->em.p, h->src.p, h->face.p, h->off.p, n_new, fresh, inew, dt, 0, h->W.soa,
-        h->n_cen, nullptr);
+    k_source_hints<<<std::min<unsigned>(grid_for(div_up(n_new, kHintStride), 256), 1024), 256, 0,
+                     h->stream>>>(h->soff.p, n, n_new, nullptr, h->src_hint.p);
   HIPCHK(h, hipGetLastError());
+  if (!source_in_transport(h, n_new)) TRY(materialize_source(h, n_new));
   HIPCHK(h, hipEventRecord(h->ev[5], h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->last_source_ms = elapsed_ms(h->ev[4], h->ev[5]);
@@ -1282,8 +1329,14 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   TRY(census_prepare(h, n, next_dt, co));
   co.inst_exit = (h->inst_exit_on && h->n_inst) ? h->inst_exit.p : nullptr;
   h->replay_timed = false;
+  // new photons not yet in W are made inside the persistent kernel
+  if (h->src_deferred && h->opt.algorithm != BH_ALGO_PERSISTENT)
+    TRY(materialize_source(h, h->n_new));
+  SourceDesc sd = h->sd;
+  sd.on = h->src_deferred ? 1 : 0;
+  h->src_deferred = false;
   HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-  TRY(launch_transport(h, h->W.soa, n, nullptr, co, handle_sinks(h, h->ctr_hi)));
+  TRY(launch_transport(h, h->W.soa, n, nullptr, co, handle_sinks(h, h->ctr_hi), sd));
   HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
   TRY(census_finish(h, n, nullptr));
   HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
@@ -1365,6 +1418,7 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
   return {u(h->W.slab),      u(h->C.slab),    (uintptr_t)h->W.cap,  (uintptr_t)h->C.cap,
           u(h->geo.p),       u(h->sigma.p),   u(h->next.p),         u(h->em.p),
           u(h->src.p),       u(h->face.p),    u(h->counts.p),       u(h->off.p),
+          u(h->soff.p),      u(h->src_hint.p),
           u(h->tiles.p),     u(h->abs_E.p),
           u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
           u(h->h_tally),     (uintptr_t)h->n_cells, (uintptr_t)h->G,
@@ -1377,24 +1431,29 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
 
 // Records source + transport + census + readback on h->stream (being
 // captured).  Sizes come from h->d_params at run time; `cap` bounds them.
-static int record_step(bh_handle* h, uint64_t cap) {
+static int record_step(bh_handle* h, uint64_t cap, bool gen) {
   const uint32_t nc = h->n_cells;
-  const DevCells cells = dev_cells(h);
   StepParams* P = h->d_params;
   HIPCHK(h, hipMemcpyAsync(P, h->h_params, sizeof(StepParams), hipMemcpyHostToDevice,
                            h->stream));
   const SourceInst unused{};
   k_source_counts<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, nc, unused,
                                                              &P->fresh, nullptr, 0, h->counts.p);
-  TRY(scan_u64(h, h->counts.p, nc, h->off.p, 1));
-  k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
-      cells, h->em.p, h->src.p, h->face.p, h->off.p, 0, unused, &P->fresh, 0.0, 0, h->W.soa, 0,
-      P);
+  TRY(scan_u64(h, h->counts.p, nc, h->soff.p, 1));
+  // the new photons: made inside the transport kernel, or written to W
+  k_source_hints<<<std::min<unsigned>(grid_for(div_up(cap, kHintStride), 256), 1024), 256, 0,
+                   h->stream>>>(h->soff.p, nc, cap, &P->n_new, h->src_hint.p);
+  if (!gen)
+    k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
+        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, 0, unused, &P->fresh, 0.0, 0,
+        h->W.soa, 0, P);
   HIPCHK(h, hipGetLastError());
+  const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, &P->fresh,
+                      unused,  0.0,      cap,       gen ? 1 : 0, 0};
   TRY(zero_tallies_and_counters(h));
   CensusOut co;  // n_cen, n_new and the census life come from P on the device
   TRY(census_prepare(h, cap, 0.0, co));
-  TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi)));
+  TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi), sd));
   TRY(census_finish(h, cap, &P->n));
   char* hp = static_cast<char*>(h->h_pinned);
   HIPCHK(h, hipMemcpyAsync(hp, h->d_ctot, sizeof(CensusPartial), hipMemcpyDeviceToHost,
@@ -1447,6 +1506,8 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
   const uint64_t pc = photon_cap(h, cap);  // as census_prepare sizes them
   TRY(ensure(h, h->counts, nc));
   TRY(ensure(h, h->off, std::max<uint64_t>(nc, pc)));
+  TRY(ensure(h, h->soff, nc));
+  TRY(ensure(h, h->src_hint, div_up(std::max<uint64_t>(cap, 1), kHintStride) + 1));
   TRY(ensure(h, h->cflag, pc));
   TRY(ensure(h, h->hrec, z.rec_cap));
   TRY(ensure(h, h->hole_len, z.rec_cap));
@@ -1480,7 +1541,9 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
                                        0, 0}};
   // one graph per parity of the photon stores (they swap every step)
   bh_handle::StepGraph& g = h->graphs[h->W.slab < h->C.slab ? 0 : 1];
-  const std::vector<uintptr_t> key = step_graph_key(h);
+  const bool gen = source_in_transport(h, n_new);
+  std::vector<uintptr_t> key = step_graph_key(h);
+  key.push_back((uintptr_t)gen);
   if (!g.exec || g.key != key) {
     if (g.exec) HIPCHK(h, hipGraphExecDestroy(g.exec));
     if (g.graph) HIPCHK(h, hipGraphDestroy(g.graph));
@@ -1488,7 +1551,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     g.graph = nullptr;
     g.key.clear();
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = record_step(h, cap);
+    const int rc = record_step(h, cap, gen);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(h->stream, &graph);
     if (rc != BH_OK) {
@@ -1606,6 +1669,7 @@ int bh_photons_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
     return fail(h, BH_ERR_CAPACITY, "%llu resident photons, buffer holds %llu",
                 (unsigned long long)total, (unsigned long long)cap);
   HIPCHK(h, hipSetDevice(h->device));
+  TRY(materialize_source(h, h->n_new));
   TRY(ensure(h, h->aos, total));
   // new photons live at W[n_cen, n_cen + n_new), census at W[0, n_cen)
   if (h->n_new)
@@ -1677,6 +1741,7 @@ int bh_census_import(bh_handle* h, const bh_photon* in, uint64_t n) {
   h->ctr_hi = hi;
   h->n_cen = n;
   h->n_new = 0;
+  h->src_deferred = false;
   return BH_OK;
 }
 
@@ -2383,10 +2448,9 @@ std::vector<uintptr_t> resident_graph_key(const bh_handle* h, uint64_t n_user_to
 // Records a resident step after the first: the step's values come in from
 // pinned memory, every count stays on the device, and one copy at the end
 // takes the report out.  cap bounds the photons of the step.
-int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0) {
+int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0, bool gen) {
   const uint32_t nc = h->n_cells, K = h->n_inst;
   const uint32_t nb = resident_blocks(h);
-  const DevCells cells = dev_cells(h);
   StepParams* P = h->d_params;
   HIPCHK(h, hipMemcpyAsync(P, h->h_params, sizeof(StepParams), hipMemcpyHostToDevice,
                            h->stream));
@@ -2401,11 +2465,18 @@ int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0) {
   SourceInst* inew = rec + (K ? K : 1);
   k_source_counts<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, nc, unused, inew,
                                                              cinst, 0, h->counts.p);
-  TRY(scan_u64(h, h->counts.p, nc, h->off.p, 1));
+  TRY(scan_u64(h, h->counts.p, nc, h->soff.p, 1));
   k_step_counts<<<1, 1, 0, h->stream>>>(P, h->d_total + 1);
-  k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
-      cells, h->em.p, h->src.p, h->face.p, h->off.p, 0, unused, inew, 0.0, 0, h->W.soa, 0, P);
+  // the new photons: made inside the transport kernel, or written to W
+  k_source_hints<<<std::min<unsigned>(grid_for(div_up(cap, kHintStride), 256), 1024), 256, 0,
+                   h->stream>>>(h->soff.p, nc, cap, &P->n_new, h->src_hint.p);
+  if (!gen)
+    k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
+        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, 0, unused, inew, 0.0, 0,
+        h->W.soa, 0, P);
   HIPCHK(h, hipGetLastError());
+  const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, inew,
+                      unused,  0.0,      cap,       gen ? 1 : 0, 0};
   if (K) {  // each instance's census photons before transport
     k_inst_census<<<std::min<unsigned>(grid_for(std::max<uint64_t>(cap, 1), 256), 4096), 256, 0,
                     h->stream>>>(h->W.soa.cell, h->W.soa.E, cap, nullptr, &P->n_cen,
@@ -2416,7 +2487,7 @@ int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0) {
   CensusOut co;
   TRY(census_prepare(h, cap, 0.0, co));
   co.inst_exit = K ? h->inst_exit.p : nullptr;
-  TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi)));
+  TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi), sd));
   TRY(census_finish(h, cap, &P->n));
   TRY(resident_finish(h, 0.0, P, nb, h->C.soa, cap, &h->d_cacc->n_census));
   char* hp = static_cast<char*>(h->h_pinned);
@@ -2457,6 +2528,8 @@ int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uin
   const uint32_t nb = resident_blocks(h);
   TRY(ensure(h, h->counts, nc));
   TRY(ensure(h, h->off, std::max<uint64_t>(nc, pc)));
+  TRY(ensure(h, h->soff, nc));
+  TRY(ensure(h, h->src_hint, div_up(std::max<uint64_t>(cap, 1), kHintStride) + 1));
   TRY(ensure(h, h->cflag, pc));
   TRY(ensure(h, h->hrec, z.rec_cap));
   TRY(ensure(h, h->hole_len, z.rec_cap));
@@ -2484,7 +2557,10 @@ int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uin
   resident_records(h, n_user, step, rank, h->h_rec);
   *h->h_params = StepParams{h->n_cen, 0, 0, kC * next_dt, dt, SourceInst{}, cap, 0u, 0u};
   bh_handle::StepGraph& g = h->resident_graphs[h->W.slab < h->C.slab ? 0 : 1];
-  const std::vector<uintptr_t> key = resident_graph_key(h, n_user_total);
+  // (the step's n_new is not known here: its n_user stands in for it)
+  const bool gen = source_in_transport(h, n_user_total);
+  std::vector<uintptr_t> key = resident_graph_key(h, n_user_total);
+  key.push_back((uintptr_t)gen);
   if (!g.exec || g.key != key) {
     if (g.exec) HIPCHK(h, hipGraphExecDestroy(g.exec));
     if (g.graph) HIPCHK(h, hipGraphDestroy(g.graph));
@@ -2492,7 +2568,7 @@ int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uin
     g.graph = nullptr;
     g.key.clear();
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = record_resident(h, cap, n_user[0]);
+    const int rc = record_resident(h, cap, n_user[0], gen);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(h->stream, &graph);
     if (rc != BH_OK) {

@@ -126,6 +126,25 @@ struct StepParams {
   uint32_t pad;
 };
 
+// New photons made where they are transported (the persistent kernel and
+// the FP64 replay) instead of being written to HBM by a source kernel and
+// read straight back: where photon p of the step's new photons comes from.
+// Photon slots from first_gen on are new photon p = slot - first_gen.
+struct SourceDesc {
+  const double* em;          // per-cell emission (initial census: census) energies
+  const double* src;         // per-cell face-source energies
+  const int8_t* face;        // per-cell SOURCE face
+  const uint64_t* off;       // exclusive prefix of the per-cell photon counts
+  const uint32_t* hint;      // hint[j]: the cell of photon j * kHintStride
+  const SourceInst* inst;    // device records (one problem: inst[0]); null: `one`
+  SourceInst one;
+  double dt;                 // (a captured step reads StepParams::dt)
+  uint64_t n_new;            // hint covers photons [0, n_new)
+  int on;                    // 0: every photon is read from the store
+  int pad;
+};
+constexpr uint32_t kHintStride = 256;
+
 struct DevCounters {
   unsigned long long segments;
   unsigned long long flushes;
@@ -361,6 +380,13 @@ __device__ __forceinline__ void source_one(const DevCells& cells,
                                            const SourceInst* __restrict__ inst, double dt,
                                            int mode, const PhotonSoA& W, uint64_t out_base);
 
+// hint[j] = the cell of new photon j * kHintStride, for j < n_hint (the
+// count read on the device when n_ptr is set): the cell of any photon p lies
+// in [hint[p / kHintStride], hint[p / kHintStride + 1]].
+__global__ void k_source_hints(const uint64_t* __restrict__ off, uint32_t n_cells,
+                               uint64_t n_new, const uint64_t* __restrict__ n_ptr,
+                               uint32_t* __restrict__ hint);
+
 // upper_bound(off[lo, hi), p) - 1: the cell whose photons include p.
 __device__ __forceinline__ uint32_t cell_of_photon(const uint64_t* __restrict__ off, uint64_t p,
                                                    uint32_t lo, uint32_t hi) {
@@ -372,6 +398,27 @@ __device__ __forceinline__ uint32_t cell_of_photon(const uint64_t* __restrict__ 
       hi = mid;
   }
   return lo - 1;
+}
+
+__global__ void k_source_hints(const uint64_t* __restrict__ off, uint32_t n_cells,
+                               uint64_t n_new, const uint64_t* __restrict__ n_ptr,
+                               uint32_t* __restrict__ hint) {
+  if (n_ptr) n_new = *n_ptr < n_new ? *n_ptr : n_new;
+  const uint64_t n_hint = (n_new + kHintStride - 1) / kHintStride;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_hint; j += stride)
+    hint[j] = cell_of_photon(off, j * kHintStride, 0u, n_cells);
+}
+
+// The cell of new photon p of n_new, from the hint array: one or two search
+// steps where a cell holds many photons.
+__device__ __forceinline__ uint32_t hinted_cell(const SourceDesc& sd, uint32_t n_cells,
+                                                uint64_t n_new, uint64_t p) {
+  const uint64_t j = p / kHintStride;
+  const uint64_t n_hint = (n_new + kHintStride - 1) / kHintStride;
+  const uint32_t lo = sd.hint[j];
+  const uint32_t hi = (j + 1 < n_hint) ? sd.hint[j + 1] + 1u : n_cells;
+  return cell_of_photon(sd.off, p, lo, hi);
 }
 
 // A block's 256 consecutive photons fall in a few consecutive cells: two

@@ -72,6 +72,7 @@ struct Live {
   double E_handoff;    // FP64 replay: the FP32 weight when FP32 stopped
   uint32_t ijk[3];     // lattice meshes: the cell's (i, j, k)
   uint8_t desc;
+  uint8_t stype;  // source type (census photons keep it)
   bool pending;  // an effective scatter awaits its draws
 };
 
@@ -175,6 +176,7 @@ __device__ __forceinline__ void load_live(const DevCells& cells, const PhotonSoA
   L.rng.ctr_hi = photon_ctr_hi(cells, L.cell, ctr_hi);
   L.group = W.group[i];
   L.desc = W.desc[i];
+  L.stype = W.stype[i];
   L.sc = 0;  // surface_cross starts at 0 for every photon (:43)
   L.segs = 0;
   L.t_abs = 0.0;
@@ -358,9 +360,10 @@ __device__ __forceinline__ bool boundary_event(const DevCells& cells, const Cell
 // reads them in order and advances its counter by the draws it uses.
 constexpr uint32_t kMaxDraws = 5;
 
+// (sized for the source draws of a wave of new photons, seven each)
 struct DrawLds {
-  uint16_t job[kWave * kMaxDraws];  // (owner lane << 3) | k
-  double res[kWave * kMaxDraws];
+  uint16_t job[kWave * kSourceDraws];  // (owner lane << 3) | k
+  double res[kWave * kSourceDraws];
 };
 
 __device__ __forceinline__ DrawLds* draw_lds() {
@@ -414,6 +417,100 @@ __device__ __forceinline__ void wave_draws(bool want, const Live<R>& L, WaveDraw
     }
   }
   wave_lds_sync();
+}
+
+// New photons made in a transport kernel: lane o with `gen` needs draws
+// 0 .. nd-1 of RNG(its seed, its stream) -- the draws the source kernel
+// would take -- and the wave computes all of them across its 64 lanes, as
+// wave_draws does for segments.  Every lane at the same program point.
+__device__ __forceinline__ void wave_gen_draws(bool gen, int nd, uint64_t stream, uint64_t hi,
+                                               WaveDraws& dr) {
+  const uint32_t K = gen ? (uint32_t)nd : 0u;
+  const uint64_t m7 = __ballot(K == 7u), m6 = __ballot(K == 6u);
+  const uint32_t off = 7u * lane_rank(m7) + 6u * lane_rank(m6);
+  const uint32_t D = 7u * (uint32_t)__popcll(m7) + 6u * (uint32_t)__popcll(m6);
+  DrawLds* s = draw_lds();
+  dr.res = s->res;
+  dr.next = off;
+  if (D == 0) return;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint32_t k = 0; k < (uint32_t)kSourceDraws; ++k)
+    if (k < K) s->job[off + k] = (uint16_t)((lane << 3) | k);
+  wave_lds_sync();
+  for (uint32_t base = 0; base < D; base += kWave) {
+    const uint32_t j = base + lane;
+    const uint32_t d = (j < D) ? (uint32_t)s->job[j] : (lane << 3);
+    const int o = (int)(d >> 3);
+    const uint64_t st = shfl_u64(stream, o);
+    const uint64_t h = shfl_u64(hi, o);
+    if (j < D) {
+      uint64_t o0, o1;
+      Threefry::block((uint64_t)(d & 7u), h, st, 0ull, o0, o1);  // counter = draw index
+      s->res[j] = u01(o0);
+    }
+  }
+  wave_lds_sync();
+}
+
+// A new photon's initial state in the registers of a tracking kernel, as
+// load_live would read it from the store.
+template <typename R, bool kLat>
+__device__ __forceinline__ void live_from_source(const DevCells& cells, const SourcePhoton& o,
+                                                 uint64_t hi, Live<R>& L) {
+  L.cell = o.cell;
+  if constexpr (kLat) lattice_ijk(cells, L.cell, L.ijk);
+  if constexpr (sizeof(R) == 4) {
+    const CellView<kLat> cv(cells, L.cell, L.ijk);
+    L.frame = L.cell;
+    L.x = (float)(o.x - cv.node(0));
+    L.y = (float)(o.y - cv.node(2));
+    L.z = (float)(o.z - cv.node(4));
+    L.life = (float)o.life;
+    L.eP = kU * (fabsf(L.x) + fabsf(L.y) + fabsf(L.z));
+    L.eT = L.eP;
+    L.eL = kU * fabsf(L.life);
+    L.eE = 0.0f;
+  } else {
+    L.x = o.x;
+    L.y = o.y;
+    L.z = o.z;
+    L.life = o.life;
+  }
+  L.u = (R)o.u;
+  L.v = (R)o.v;
+  L.w = (R)o.w;
+  L.E = o.E;
+  L.E0 = o.E;
+  L.rng.ctr = o.ctr;
+  L.rng.stream = o.stream;
+  L.rng.ctr_hi = hi;
+  L.group = o.group;
+  L.desc = kPass;
+  L.stype = o.stype;
+  L.sc = 0;
+  L.segs = 0;
+  L.t_abs = 0.0;
+  L.t_track = 0.0;
+  L.first_seg = 0;
+  L.pending = false;
+}
+
+// New photon p made by one lane alone (the FP64 replay): its draws one by
+// one, then the source's sampling.
+template <typename R, bool kLat>
+__device__ __forceinline__ void gen_photon_lane(const DevCells& cells, const SourceDesc& sd,
+                                                uint64_t n_new, double dt, uint64_t p,
+                                                Live<R>& L) {
+  const uint32_t c = hinted_cell(sd, cells.n_cells, n_new, p);
+  const SourceInst sp = source_record(cells, c, sd.one, sd.inst);
+  const int nd = source_draw_count(sd.em, sd.off, p, c, sp, 0);
+  Rng rng{0ull, source_stream(sp, sd.off, p), sp.ctr_hi};
+  double u[kSourceDraws];
+#pragma unroll
+  for (int k = 0; k < kSourceDraws; ++k) u[k] = (k < nd) ? rng.next() : 0.0;
+  const SourcePhoton o = make_source_photon(cells, sd.em, sd.src, sd.face, sd.off, p, c, sp, dt,
+                                            0, u);
+  live_from_source<R, kLat>(cells, o, sp.ctr_hi, L);
 }
 
 // Draw states of the lane-by-lane form of a segment's draws.
@@ -981,6 +1078,12 @@ constexpr uint32_t kCensusArgOffset =
     (uint32_t)((sizeof(PhotonSoA) + alignof(CensusOut) - 1) / alignof(CensusOut) *
                alignof(CensusOut));
 
+// The kernels that make new photons take the SourceDesc third, after the
+// census output.
+constexpr uint32_t kSourceArgOffset =
+    (uint32_t)((kCensusArgOffset + sizeof(CensusOut) + alignof(SourceDesc) - 1) /
+               alignof(SourceDesc) * alignof(SourceDesc));
+
 // (The host pass parses these device functions too; it has no kernel
 // argument segment, and never runs them.)
 __device__ __forceinline__ PhotonSoA arg_stores() {
@@ -1002,6 +1105,16 @@ __device__ __forceinline__ CensusOut arg_census() {
 #endif
 }
 
+__device__ __forceinline__ SourceDesc arg_source() {
+#if __HIP_DEVICE_COMPILE__
+  KernargPtr k = (KernargPtr)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(k));
+  return *(const __attribute__((address_space(4))) SourceDesc*)(k + kSourceArgOffset);
+#else
+  return SourceDesc{};
+#endif
+}
+
 // The view must show the by-value arguments.  Every work-item of a launch
 // reaches the same answer; a launch that fails it does nothing, and the
 // host reports the count.
@@ -1014,6 +1127,14 @@ __device__ __forceinline__ bool args_ok(const PhotonSoA& W, const CensusOut& co,
                   c.slots == co.slots && c.rec_cap == co.rec_cap && c.on == co.on;
   if (!ok && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctrs->bad_args, 1ull);
   return ok;
+}
+__device__ __forceinline__ bool args_ok(const PhotonSoA& W, const CensusOut& co,
+                                        const SourceDesc& sd, DevCounters* ctrs) {
+  const SourceDesc v = arg_source();
+  const bool ok = v.off == sd.off && v.hint == sd.hint && v.em == sd.em && v.on == sd.on &&
+                  v.n_new == sd.n_new;
+  if (!ok && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctrs->bad_args, 1ull);
+  return ok && args_ok(W, co, ctrs);
 }
 
 // A launch's counts: the kernel argument's, or the captured step's.
@@ -1088,7 +1209,7 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
       co.C.life[s] = cr.life;
       co.C.E0[s] = L.E0;
       co.C.stream[s] = L.rng.stream;
-      co.C.stype[s] = W.stype[i];
+      co.C.stype[s] = L.stype;
       co.C.key[s] = (uint32_t)t;
       // the census energy by transport index marks the photon (-0.0 for a
       // zero weight: nonzero bits, adds nothing); every key is set by its
@@ -1188,16 +1309,20 @@ BH_TRACK_KERNEL void k_transport_history(const PhotonSoA W, const CensusOut co,
 // claimed with one atomic on the global queue head, so the queue sees one
 // atomic per chunk histories instead of one per refill.  The loop ends
 // when the queue is drained and no lane holds a photon, so every wave exits.
-template <class Tally, typename R, bool kLat>
+// With sd.on, the slots from the census count on are the step's new
+// photons, made here when a lane takes them (the source's draws computed by
+// the whole wave, then the source's sampling) instead of being read.
+template <class Tally, typename R, bool kLat, bool kGen>
 BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut co,
-                                            DevCells cells, uint64_t n,
+                                            const SourceDesc sd, DevCells cells, uint64_t n,
                                             uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
                                             PersistentQueue* q, int refill, uint32_t chunk,
                                             ReplayList rl, const StepParams* P) {
-  if (!args_ok(W, co, ctrs)) return;
+  if (!args_ok(W, co, sd, ctrs)) return;
   const uint64_t kChunk = chunk;
   if (P) n = P->n;  // a step captured as a graph
   const CensusRun cr = census_run(co, P);
+  const uint64_t first_gen = (kGen && sd.on) ? cr.n_cen : ~0ull;
   CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
@@ -1205,7 +1330,7 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
   Live<R> L;
   L.rng = Rng{0, 0, 0};
   L.pending = false;
-  bool active = false;
+  bool active = false, gen = false;
   uint64_t idx = 0;
   uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
   bool drained = false;
@@ -1234,13 +1359,40 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
         const bool take = !active && rank < avail;
         if (take) {
           idx = chunk_next + rank;
-          load_live<R, kLat>(cells, arg_stores(), idx, ctr_hi, L);
+          gen = kGen && idx >= first_gen;
+          if (!gen) load_live<R, kLat>(cells, arg_stores(), idx, ctr_hi, L);
           active = true;
         }
         const uint64_t taken = (avail < (uint64_t)n_idle) ? avail : (uint64_t)n_idle;
         chunk_next += taken;
         idle = __ballot(!active);
         n_idle = __popcll(idle);
+      }
+      if (kGen && __ballot(gen) != 0) {  // the new photons this refill took
+        const SourceDesc s = arg_source();
+        uint64_t p = 0, stream = 0, hi = 0;
+        uint32_t c = 0;
+        int nd = 0;
+        SourceInst sp{};
+        if (gen) {
+          p = idx - first_gen;
+          c = hinted_cell(s, cells.n_cells, cr.n_new, p);
+          sp = source_record(cells, c, s.one, s.inst);
+          nd = source_draw_count(s.em, s.off, p, c, sp, 0);
+          stream = source_stream(sp, s.off, p);
+          hi = sp.ctr_hi;
+        }
+        WaveDraws gd;
+        wave_gen_draws(gen, nd, stream, hi, gd);
+        if (gen) {
+          double u[kSourceDraws];
+#pragma unroll
+          for (int k = 0; k < kSourceDraws; ++k) u[k] = (k < nd) ? gd.res[gd.next + k] : 0.0;
+          const SourcePhoton o = make_source_photon(cells, s.em, s.src, s.face, s.off, p, c, sp,
+                                                    P ? P->dt : s.dt, 0, u);
+          live_from_source<R, kLat>(cells, o, hi, L);
+        }
+        gen = false;
       }
     }
     if (__ballot(active) == 0) break;
@@ -1437,10 +1589,12 @@ __global__ __launch_bounds__(256) void k_ev_scatter(PhotonSoA W, DevCells cells,
 // launches.  Loop bounds are block-uniform.
 template <class Tally>
 BH_TRACK_KERNEL void k_transport_replay(const PhotonSoA W, const CensusOut co,
-                                        DevCells cells, ReplayList rl, uint64_t ctr_hi,
-                                        Tally tally, DevCounters* ctrs, const StepParams* P) {
-  if (!args_ok(W, co, ctrs)) return;
+                                        const SourceDesc sd, DevCells cells, ReplayList rl,
+                                        uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
+                                        const StepParams* P) {
+  if (!args_ok(W, co, sd, ctrs)) return;
   const CensusRun cr = census_run(co, P);
+  const uint64_t first_gen = sd.on ? cr.n_cen : ~0ull;
   CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
@@ -1454,7 +1608,11 @@ BH_TRACK_KERNEL void k_transport_replay(const PhotonSoA W, const CensusOut co,
     Live<double> L;
     if (active) {
       i = rl.idx[j];
-      load_live(cells, arg_stores(), i, ctr_hi, L);
+      if (i >= first_gen)  // a new photon: made again from its draws
+        gen_photon_lane<double, false>(cells, arg_source(), cr.n_new, P ? P->dt : sd.dt,
+                                       i - first_gen, L);
+      else
+        load_live(cells, arg_stores(), i, ctr_hi, L);
       L.first_seg = rl.first_seg[j];
       L.E_handoff = rl.E[j];
     }

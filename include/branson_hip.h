@@ -94,7 +94,11 @@ typedef struct bh_options {
                                group -- any deck-built mesh) is tracked
                                from compact per-cell records and axis
                                arrays; 1: always the general cell records */
-  int32_t reserved;         /* set to 0                                    */
+  int32_t source_mode;      /* where the persistent algorithm's new photons
+                               are made: 0 auto (= 2, measured faster), 1
+                               inside the transport kernel as each is taken
+                               (no HBM round trip of their state), 2 by the
+                               source kernel into the photon store        */
 } bh_options;
 
 /* Read-only cell data for one step (src/cell.h:319-336). */

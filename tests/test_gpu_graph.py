@@ -35,13 +35,16 @@ def test_graph_driver_reproduces_reference_marshak_run():
             assert _close6(r[k], g[k]), (r["step"], k, r[k], g[k])
 
 
+@pytest.mark.parametrize("source_mode", [capi.SOURCE_AUTO, capi.SOURCE_IN_TRANSPORT])
 @pytest.mark.parametrize("prec", [capi.PRECISION_FP64, capi.PRECISION_FP32_TRACK])
-def test_graph_steps_equal_eager_steps(prec):
+def test_graph_steps_equal_eager_steps(prec, source_mode):
+    """source_mode SOURCE_IN_TRANSPORT: the graph's new photons are made
+    inside the transport kernel, the eager steps' by the source kernel."""
     deck = os.path.join(GOLDEN, "decks", "cube_40.xml")
     out = {}
     for graph in (False, True):
         with capi.Driver(deck, algorithm=capi.ALGO_PERSISTENT, precision=prec, t_stop=0.005,
-                         step_graph=graph) as d:
+                         step_graph=graph, source_mode=source_mode if graph else 0) as d:
             steps = []
             while not d.finished():
                 d.prepare()

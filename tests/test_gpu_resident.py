@@ -38,12 +38,16 @@ def _rel(a, b):
     return float(np.max(d / s)) if d.size else 0.0
 
 
-@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT])
+@pytest.mark.parametrize("algo,source_mode", [(capi.ALGO_HISTORY, 0), (capi.ALGO_PERSISTENT, 0),
+                                              (capi.ALGO_PERSISTENT, capi.SOURCE_IN_TRANSPORT)])
 @pytest.mark.parametrize("name", ["marshak_slab_100", "cube_40"])
-def test_resident_driver_reproduces_reference_run(name, algo):
+def test_resident_driver_reproduces_reference_run(name, algo, source_mode):
+    """Persistent: steps after the first are one graph launch each; with
+    SOURCE_IN_TRANSPORT their new photons are made inside the transport
+    kernel."""
     run = json.load(open(os.path.join(GOLDEN, "reference_runs.json")))["runs"][name]
     with capi.Driver(os.path.join(GOLDEN, run["deck"]), algorithm=algo,
-                     device_physics=True) as d:
+                     device_physics=True, source_mode=source_mode) as d:
         reps = d.run()
     assert len(reps) == len(run["steps"])
     for r, g in zip(reps, run["steps"]):
