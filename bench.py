@@ -1,7 +1,7 @@
 """Benchmark: replicated IMC transport phase on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--workload headline|cube|hohlraum|batched|adjoint] [--fp32-tol T]
+                    [--workload headline|cube|batched|adjoint|precision] [--fp32-tol T]
 
 Default workload (headline): the 30-group hohlraum-like configuration -- 64^3
 cells of 0.01 cm, a 4-cell wall (sigma_a = 1000/cm, rho = 19) around a
@@ -82,7 +82,7 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
             w.run()
     out = dict(hist=0, segs=0, flushes=0, dev_s=0.0, step_s=0.0, k_ms=[], launches=[],
                step1=None, replayed=0, lattice=None, tally=None, src_ms=[], cen_ms=[],
-               replay_ms=[])
+               replay_ms=[], allocs=0)
     with capi.Driver(deck, t_stop=DT * steps, reserve_photons=reserve, **kw) as d:
         eng = capi.lib()
         h = eng.bh_driver_engine(d._d)
@@ -111,6 +111,7 @@ def _run_form(capi, steps, warmup, precision, algorithm, keep_step1=False, deck=
             out["src_ms"].append(r["source_ms"])
             out["cen_ms"].append(r["census_ms"])
             out["replay_ms"].append(r["replay_ms"])
+            out["allocs"] += r["n_allocs"]
             out["launches"].append((r["transported"], r["n_segments"], r["n_flushes"]))
     return out
 
@@ -299,6 +300,7 @@ def _bench_headline(a, capi, cube=True):
         "meets_tally_bar": fp32_ok,
         "replayed_histories": fp32["replayed"],
         "replay_fraction": fp32["replayed"] / max(fp32["hist"], 1),
+        "device_allocations_in_timed_steps": fp32["allocs"],
         "kernel_ms_per_step": dict(_kernel_split(fp32),
                                    fp32_pass_ms=_mean(fp32["k_ms"]) - _mean(fp32["replay_ms"]),
                                    fp64_replay_ms=_mean(fp32["replay_ms"])),
@@ -323,7 +325,8 @@ def _bench_headline(a, capi, cube=True):
         plain_fp64={"histories_per_s": plain_rate,
                     "segments_per_s": plain["segs"] / plain["dev_s"],
                     "ms_per_step": 1e3 * plain["dev_s"] / a.steps,
-                    "kernel_ms_per_step": _kernel_split(plain)},
+                    "kernel_ms_per_step": _kernel_split(plain),
+                    "device_allocations_in_timed_steps": plain["allocs"]},
         fp32_tracking=fp32_block,
         graph_steps={"histories_per_s": graph["hist"] / graph["dev_s"],
                      "ms_per_step": 1e3 * graph["dev_s"] / a.steps},
@@ -451,6 +454,42 @@ def _bench_adjoint(a, capi):
                               "fleck": float(np.linalg.norm(df))})
 
 
+def _bench_precision(a, capi):
+    """FP32 tracking against plain FP64 on the 30-group hohlraum at 1e7
+    photons per step (2 steps each): per-step transport and FP64-replay time,
+    replayed histories and the step-1 per-cell tally error, at fp32_tol 1e-6
+    and 1e-2.  The command the VALU-counter profile of the two forms runs."""
+    forms = [("plain fp64", capi.PRECISION_FP64, 0.0),
+             ("fp32 tracking, fp32_tol 1e-6", capi.PRECISION_FP32_TRACK, 1e-6),
+             ("fp32 tracking, fp32_tol 1e-2", capi.PRECISION_FP32_TRACK, 1e-2)]
+    out, ref = [], None
+    for name, prec, tol in forms:
+        with capi.Driver(HOHLRAUM, n_groups=30, n_photons=10_000_000, t_stop=2 * DT,
+                         algorithm=capi.ALGO_PERSISTENT, precision=prec, fp32_tol=tol,
+                         reserve_photons=20_000_000) as d:
+            reps, tal = [], None
+            while not d.finished():
+                d.prepare()
+                d.transport()
+                reps.append(d.finish())
+                if tal is None:
+                    tal = d.tallies()[0]
+        if ref is None:
+            ref = tal
+        out.append({"form": name, "histories": sum(r["transported"] for r in reps),
+                    "replayed": sum(r["n_replayed"] for r in reps),
+                    "transport_ms_per_step": _mean([r["transport_ms"] for r in reps]),
+                    "fp64_replay_ms_per_step": _mean([r["replay_ms"] for r in reps]),
+                    "step1_tally_error_vs_fp64": _tally_error(ref, tal)})
+    f64 = out[0]
+    ms = f64["transport_ms_per_step"]
+    return _common("hohlraum histories/s of the transport kernel, plain FP64 beside FP32 "
+                   "tracking", f64["histories"] / 2 / (ms * 1e-3), 2, 0, ms, "f64",
+                   {"workload": "precision study: hohlraum_64, 30 groups, 1e7 photons/step, "
+                                "2 steps per form", "form": "plain fp64"},
+                   forms=out)
+
+
 def _bench_cube(a, capi):
     """The 40^3 cube throughput configuration alone, plain FP64, with the
     one-work-item-per-history form and FP32 tracking beside it."""
@@ -495,10 +534,11 @@ def main(argv=None):
                     help="skip the resident whole-step leg (the other legs' launches are "
                          "unchanged; profiling runs use it)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "cube", "batched", "adjoint"],
+                    choices=["headline", "cube", "batched", "adjoint", "precision"],
                     help="headline (default): the 30-group hohlraum, plain FP64 against "
                          "FP32 tracking, with the cube throughput beside it; cube: the cube "
-                         "alone; batched: the 256-problem batch; adjoint: the adjoint")
+                         "alone; batched: the 256-problem batch; adjoint: the adjoint; "
+                         "precision: FP32 tracking against FP64 on the hohlraum")
     ap.add_argument("--fp32-tol", type=float, default=0.0,
                     help="FP32 tracking: per-deposit relative error bound (0 = default 1e-6)")
     a = ap.parse_args(argv)
@@ -513,7 +553,7 @@ def main(argv=None):
     from branson_amd import capi
 
     fn = {"headline": _bench_headline, "cube": _bench_cube, "batched": _bench_batched,
-          "adjoint": _bench_adjoint}[a.workload]
+          "adjoint": _bench_adjoint, "precision": _bench_precision}[a.workload]
     print(json.dumps(fn(a, capi)), flush=True)
 
 

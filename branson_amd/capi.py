@@ -74,7 +74,7 @@ class StepStats(C.Structure):
                 ("n_flushes", C.c_uint64), ("transport_ms", C.c_double),
                 ("census_ms", C.c_double), ("source_ms", C.c_double),
                 ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("tally", C.c_int32),
-                ("replay_ms", C.c_double)]
+                ("replay_ms", C.c_double), ("n_allocs", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -113,6 +113,7 @@ class CombReport(C.Structure):
 
 
 _lib = None
+ABI_VERSION = 4  # BH_ABI_VERSION of include/branson_hip.h
 
 
 def lib() -> C.CDLL:
@@ -159,6 +160,9 @@ def lib() -> C.CDLL:
     for name in EXPORTS:
         if name not in ("bh_destroy", "bh_last_error", "bh_abi_version"):
             getattr(L, name).restype = I
+    if L.bh_abi_version() != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI version {L.bh_abi_version()}, these bindings "
+                          f"expect {ABI_VERSION}: rebuild it with `python -m branson_amd.build`")
     _lib = L
     return L
 
@@ -466,7 +470,7 @@ class DriverReport(C.Structure):
                 ("census_ms", C.c_double), ("prepare_ms", C.c_double),
                 ("device_phase_ms", C.c_double), ("finish_ms", C.c_double),
                 ("n_replayed", C.c_uint64), ("lattice", C.c_int32), ("tally", C.c_int32),
-                ("replay_ms", C.c_double)]
+                ("replay_ms", C.c_double), ("n_allocs", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

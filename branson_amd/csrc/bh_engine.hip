@@ -133,6 +133,7 @@ struct bh_handle {
   bool lattice = false;          // mesh is a lattice: CellRec + axis arrays
   double src_neff_frac = 1.0;    // effective source cells / n_cells (tally choice)
   int last_tally = 0;            // tally form of the last launch (bh_step_stats.tally)
+  uint64_t n_allocs = 0;         // device allocations of step calls since the last stats report
 
   // device-resident step physics (bh_set_material / bh_step_resident)
   bool has_material = false;
@@ -267,6 +268,23 @@ int fail(bh_handle* h, int code, const char* fmt, ...) {
     if (r_ != BH_OK) return r_; \
   } while (0)
 
+// Set-up, export and probe calls allocate as set-up, not as step work:
+// their allocations are left out of bh_step_stats.n_allocs.
+struct SetupAllocs {
+  bh_handle* h;
+  uint64_t n;
+  explicit SetupAllocs(bh_handle* hh) : h(hh), n(hh ? hh->n_allocs : 0) {}
+  ~SetupAllocs() {
+    if (h) h->n_allocs = n;
+  }
+};
+
+uint64_t take_allocs(bh_handle* h) {
+  const uint64_t n = h->n_allocs;
+  h->n_allocs = 0;
+  return n;
+}
+
 // Makes a hold at least n elements (contents are not kept).  A growing
 // array grows by half again, so one that follows a growing census is
 // reallocated rarely: every hipMalloc / hipFree synchronises the device.
@@ -277,6 +295,7 @@ int ensure(bh_handle* h, DArr<T>& a, size_t n) {
   if (a.p) HIPCHK(h, hipFree(a.p));
   a.p = nullptr;
   a.cap = 0;
+  ++h->n_allocs;
   if (hipMalloc(&a.p, want * sizeof(T)) != hipSuccess) {
     a.p = nullptr;
     return fail(h, BH_ERR_ALLOC, "device allocation of %zu bytes failed", want * sizeof(T));
@@ -299,6 +318,7 @@ int grow_store(bh_handle* h, PhotonStore& s, uint64_t need, uint64_t keep) {
   // rarely (each reallocation is a device-wide synchronising hipMalloc)
   uint64_t cap = std::max<uint64_t>(need + need / 2, 1024);
   void* slab = nullptr;
+  ++h->n_allocs;
   if (hipMalloc(&slab, PhotonStore::bytes_for(cap)) != hipSuccess)
     return fail(h, BH_ERR_ALLOC, "device allocation for %llu photons failed",
                 (unsigned long long)cap);
@@ -860,6 +880,48 @@ float elapsed_ms(hipEvent_t a, hipEvent_t b) {
 // ===========================================================================
 extern "C" {
 
+namespace {
+// The photon stores for launches of up to `n` photons, and every buffer a
+// step sizes from the stores' capacity (census flags and ranks, scan tiles,
+// hole records, source hints, the FP32 replay list, the event-split lists),
+// allocated together at creation: steps within the reservation then
+// allocate nothing.  Each hipMalloc / hipFree synchronises the device, and
+// multi-GB ones inside a step cost far more than its kernels.
+int reserve_buffers(bh_handle* h, uint64_t n) {
+  TRY(grow_store(h, h->W, n, 0));
+  TRY(grow_store(h, h->C, census_sizes(h, n).slots, 0));
+  const uint64_t pc = photon_cap(h, n);
+  const CensusSizes z = census_sizes(h, pc);
+  const uint64_t tiles = std::max<uint64_t>(div_up(pc, kScanTile), 1);
+  TRY(ensure(h, h->cflag, pc));
+  TRY(ensure(h, h->off, pc));
+  TRY(ensure(h, h->tiles, tiles));
+  TRY(ensure(h, h->tile_E, tiles));
+  TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
+  TRY(ensure(h, h->hrec, z.rec_cap));
+  TRY(ensure(h, h->hole_len, z.rec_cap));
+  TRY(ensure(h, h->hole_off, z.rec_cap));
+  TRY(ensure(h, h->tail_live, z.hole_cap));
+  TRY(ensure(h, h->tail_rank, z.hole_cap));
+  TRY(ensure(h, h->src_hint, div_up(pc, kHintStride) + 1));
+  if (h->opt.precision == BH_PRECISION_FP32_TRACK) {
+    TRY(ensure(h, h->rep_idx, pc));
+    TRY(ensure(h, h->rep_first, pc));
+    TRY(ensure(h, h->rep_E, pc));
+  }
+  if (h->opt.algorithm == BH_ALGO_EVENT) {
+    TRY(ensure(h, h->ev_abs, pc));
+    TRY(ensure(h, h->ev_track, pc));
+    TRY(ensure(h, h->ev_segs, pc));
+    TRY(ensure(h, h->ev_sc, pc));
+    TRY(ensure(h, h->ev_list0, pc));
+    TRY(ensure(h, h->ev_list1, pc));
+    TRY(ensure(h, h->ev_scatter, pc));
+  }
+  return BH_OK;
+}
+}  // namespace
+
 int bh_abi_version(void) { return BH_ABI_VERSION; }
 
 int bh_create(int device, const bh_options* opts, bh_handle** out) {
@@ -916,6 +978,9 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
   HIPCHK(h, hipMalloc(&h->d_ctrs, sizeof(DevCounters)));
   HIPCHK(h, hipMalloc(&h->d_queue, sizeof(PersistentQueue)));
   HIPCHK(h, hipMalloc(&h->d_ev_count, 4 * sizeof(unsigned long long)));
+  HIPCHK(h, hipMalloc(&h->d_params, sizeof(StepParams)));
+  HIPCHK(h, hipMalloc(&h->d_phys, sizeof(PhysTotals)));
+  HIPCHK(h, hipMalloc(&h->d_inst2, 2 * sizeof(SourceInst)));
   HIPCHK(h, hipHostMalloc(&h->h_pinned, 4096, hipHostMallocDefault));
   {  // the no-allocation entry point's device memory on this device
     const hipError_t e = scratch_on_device(device, &h->scratch);
@@ -924,10 +989,8 @@ int bh_create(int device, const bh_options* opts, bh_handle** out) {
       h->scratch_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
     }
   }
-  if (h->opt.reserve_photons) {
-    TRY(grow_store(h, h->W, h->opt.reserve_photons, 0));
-    TRY(grow_store(h, h->C, h->opt.reserve_photons, 0));
-  }
+  if (h->opt.reserve_photons) TRY(reserve_buffers(h, h->opt.reserve_photons));
+  h->n_allocs = 0;
   return BH_OK;
 }
 
@@ -1030,6 +1093,7 @@ int bh_sync(bh_handle* h) {
 }
 
 int bh_set_mesh(bh_handle* h, const bh_cells* c) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream) return BH_ERR_ARG;
   TRY(check_cells(h, c));
   if (h->has_mesh) TRY(materialize_source(h, h->n_new));  // they read the old cells
@@ -1057,6 +1121,9 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
   TRY(ensure(h, h->sigma, (size_t)n * G));
   TRY(ensure(h, h->abs_E, n));
   TRY(ensure(h, h->track_E, n));
+  TRY(ensure(h, h->counts, n));  // the source's per-cell counts and offsets
+  TRY(ensure(h, h->soff, n));
+  TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(n, kScanTile), 1)));
   HIPCHK(h, hipMemcpyAsync(h->geo.p, geo.data(), n * sizeof(CellGeo), hipMemcpyHostToDevice,
                            h->stream));
   HIPCHK(h, hipMemcpyAsync(h->next.p, c->next, (size_t)n * 6 * sizeof(uint32_t),
@@ -1105,6 +1172,7 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
 
 int bh_set_source_energy(bh_handle* h, const double* emission_E, const double* source_E,
                          const double* census_E, const int8_t* source_face) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_source_energy before bh_set_mesh");
   // photons of an earlier bh_source still to be made read these energies
@@ -1229,6 +1297,7 @@ int bh_source(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_t se
 
 int bh_set_instances(bh_handle* h, uint32_t n_instances, const uint32_t* first_cell,
                      const uint32_t* seeds) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_instances before bh_set_mesh");
   HIPCHK(h, hipSetDevice(h->device));
@@ -1386,6 +1455,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
     stats->n_replayed = ctrs.replays;
     stats->lattice = h->lattice ? 1 : 0;
     stats->tally = h->last_tally;
+    stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     stats->census_ms = elapsed_ms(h->ev[1], h->ev[2]);
     stats->source_ms = h->last_source_ms;
@@ -1525,7 +1595,6 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     TRY(ensure(h, h->rep_first, rep));
     TRY(ensure(h, h->rep_E, rep));
   }
-  if (!h->d_params) HIPCHK(h, hipMalloc(&h->d_params, sizeof(StepParams)));
   if (!h->h_params) HIPCHK(h, hipHostMalloc(&h->h_params, sizeof(StepParams), hipHostMallocDefault));
   if (h->h_tally_cap < 2 * (size_t)nc) {
     if (h->h_tally) HIPCHK(h, hipHostFree(h->h_tally));
@@ -1611,6 +1680,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     stats->n_replayed = ctrs.replays;
     stats->lattice = h->lattice ? 1 : 0;
     stats->tally = h->last_tally;
+    stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph
   }
   if (ctrs.stuck)
@@ -1623,6 +1693,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
 }
 
 int bh_get_tallies(bh_handle* h, double* abs_E, double* track_E) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "no mesh");
   if (abs_E)
@@ -1642,6 +1713,7 @@ int bh_census_size(bh_handle* h, uint64_t* n) {
 }
 
 int bh_census_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || !n) return BH_ERR_ARG;
   *n = h->n_cen;
   if (h->n_cen == 0) return BH_OK;
@@ -1661,6 +1733,7 @@ int bh_census_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
 }
 
 int bh_photons_export(bh_handle* h, bh_photon* out, uint64_t cap, uint64_t* n) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || !n) return BH_ERR_ARG;
   const uint64_t total = h->n_cen + h->n_new;
   *n = total;
@@ -1734,6 +1807,7 @@ static int upload_aos(bh_handle* h, const bh_photon* in, uint64_t n, PhotonStore
 }
 
 int bh_census_import(bh_handle* h, const bh_photon* in, uint64_t n) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || (n && !in)) return BH_ERR_ARG;
   HIPCHK(h, hipSetDevice(h->device));
   uint64_t hi = 0;
@@ -1825,6 +1899,7 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     stats->n_replayed = ctrs.replays;
     stats->lattice = h->lattice ? 1 : 0;
     stats->tally = h->last_tally;
+    stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     outcome_stats(photons, n, stats);
   }
@@ -1952,6 +2027,7 @@ int bh_transport_photons_noalloc(bh_handle* h, bh_photon* photons, uint64_t n, d
     stats->n_replayed = ctrs.replays;
     stats->lattice = h->lattice ? 1 : 0;
     stats->tally = h->last_tally;
+    stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     outcome_stats(photons, n, stats);
   }
@@ -1964,6 +2040,7 @@ int bh_transport_photons_noalloc(bh_handle* h, bh_photon* photons, uint64_t n, d
 int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
                        const double* abs_adjoint, double* d_sigma_a, double* d_fleck,
                        bh_step_stats* stats) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || (n && !photons) || !abs_adjoint || !d_sigma_a || !d_fleck)
     return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_adjoint_photons before bh_set_mesh");
@@ -2026,6 +2103,7 @@ int bh_adjoint_photons(bh_handle* h, const bh_photon* photons, uint64_t n,
 // ---- census population control ------------------------------------------------
 int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint64_t stream,
                    uint64_t ctr0, bh_comb_report* rep) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_comb_census before bh_set_mesh");
   if (max_census_photons <= 0) return fail(h, BH_ERR_ARG, "max_census_photons must be > 0");
@@ -2040,7 +2118,6 @@ int bh_comb_census(bh_handle* h, int64_t max_census_photons, uint32_t seed, uint
   // E_census: deterministic parallel sum, with the bound on the serial loop's
   const uint32_t nb = std::max<uint32_t>(1u, std::min<uint32_t>(kSumMaxBlocks, div_up(n, 4096)));
   TRY(ensure(h, h->sum_part, (size_t)8 * nb));
-  if (!h->d_phys) HIPCHK(h, hipMalloc(&h->d_phys, sizeof(PhysTotals)));
   k_energy_partials<<<nb, kSumNT, 0, h->stream>>>(h->W.soa.E, n, h->sum_part.p);
   k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, nb, &h->d_phys->global_E);
   HIPCHK(h, hipGetLastError());
@@ -2162,6 +2239,7 @@ static MatDev mat_dev(bh_handle* h) {
 
 int bh_set_material(bh_handle* h, const bh_material* m, double replicated_factor,
                     uint32_t n_ranks, uint32_t rank) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || !m) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_set_material before bh_set_mesh");
   if (h->n_inst && (n_ranks != 1 || replicated_factor != 1.0))
@@ -2188,9 +2266,6 @@ int bh_set_material(bh_handle* h, const bh_material* m, double replicated_factor
   TRY(ensure(h, h->src, n));
   TRY(ensure(h, h->cen, n));
   TRY(ensure(h, h->face, n));
-  if (!h->d_phys) HIPCHK(h, hipMalloc(&h->d_phys, sizeof(PhysTotals)));
-  if (!h->d_inst2) HIPCHK(h, hipMalloc(&h->d_inst2, 2 * sizeof(SourceInst)));
-  if (!h->d_params) HIPCHK(h, hipMalloc(&h->d_params, sizeof(StepParams)));
   if (!h->h_params)
     HIPCHK(h, hipHostMalloc(&h->h_params, sizeof(StepParams), hipHostMallocDefault));
   for (int k = 0; k < 9; ++k)
@@ -2621,6 +2696,7 @@ int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uin
   st.n_replayed = ctrs.replays;
   st.lattice = h->lattice ? 1 : 0;
   st.tally = h->last_tally;
+  st.n_allocs = take_allocs(h);
   st.transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph
   fill_report(h, pt, st, Pd.n, Pd.n_new, false, rep);
   if (K) fill_instance_reports(h, h->h_inst_rep, rep, irep);
@@ -2691,6 +2767,7 @@ int bh_step_resident_instances(bh_handle* h, const uint64_t* n_user_photons, uin
 }
 
 int bh_get_temperature(bh_handle* h, double* T_e, double* T_r) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_material) return fail(h, BH_ERR_STATE, "no material (bh_set_material)");
   const size_t n = h->n_cells;
@@ -2706,6 +2783,7 @@ int bh_get_temperature(bh_handle* h, double* T_e, double* T_r) {
 
 int bh_get_physics(bh_handle* h, double* sigma_a, double* sigma_s, double* fleck,
                    double* emission_E, double* source_E) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream) return BH_ERR_ARG;
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "no mesh");
   const size_t n = h->n_cells, G = h->G;
@@ -2738,6 +2816,7 @@ int bh_get_physics(bh_handle* h, double* sigma_a, double* sigma_s, double* fleck
 
 int bh_rng_draws(bh_handle* h, uint32_t seed, uint64_t stream, uint64_t ctr0, uint32_t n,
                  double* out) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || (n && !out)) return BH_ERR_ARG;
   if (n == 0) return BH_OK;
   if (ctr0 + n < ctr0)
@@ -2758,6 +2837,7 @@ int bh_rng_draws(bh_handle* h, uint32_t seed, uint64_t stream, uint64_t ctr0, ui
 }
 
 int bh_copy_probe(bh_handle* h, uint64_t n_bytes, int reps, double* gbps) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || !gbps || reps <= 0 || n_bytes < 16) return BH_ERR_ARG;
   HIPCHK(h, hipSetDevice(h->device));
   const uint64_t n = n_bytes / 16;
@@ -2788,6 +2868,7 @@ int bh_copy_probe(bh_handle* h, uint64_t n_bytes, int reps, double* gbps) {
 }
 
 int bh_threefry(bh_handle* h, const uint64_t* in, uint64_t n, uint64_t* out) {
+  SetupAllocs setup_allocs(h);
   if (!h || !h->stream || (n && (!in || !out))) return BH_ERR_ARG;
   if (n == 0) return BH_OK;
   HIPCHK(h, hipSetDevice(h->device));

@@ -645,6 +645,7 @@ int bh_driver_finish(bh_driver* d, bh_driver_report* rep) {
   R.transport_ms = d->stats.transport_ms;
   R.census_ms = d->stats.census_ms;
   R.replay_ms = d->stats.replay_ms;
+  R.n_allocs = d->stats.n_allocs;
   R.prepare_ms = d->prepare_ms;
   R.device_phase_ms = d->device_ms;
   d->prev_census_E = d->stats.census_E;

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BH_ABI_VERSION 3
+#define BH_ABI_VERSION 4
 
 /* return codes */
 #define BH_OK 0
@@ -79,7 +79,10 @@ typedef struct bh_options {
                             otherwise as 1)                               */
   int32_t refill_threshold; /* persistent: idle lanes of a wave that trigger
                                a refill (1..64), 0 = default (16)          */
-  uint64_t reserve_photons; /* initial photon capacity hint, 0 = none      */
+  uint64_t reserve_photons; /* photons per launch that the photon stores and
+                               every step buffer are sized for at bh_create
+                               (steps within it allocate nothing); 0: grow
+                               on demand                                  */
   double fp32_tol;          /* FP32 tracking: largest relative error bound a
                                single deposit (and the photon weight) may
                                carry; a history that would exceed it is
@@ -137,6 +140,10 @@ typedef struct bh_step_stats {
                              global atomics per flush, 4 wave hash table   */
   double replay_ms;       /* FP32 tracking: device time of the FP64 replay,
                              included in transport_ms (0 in FP64)          */
+  uint64_t n_allocs;      /* device allocations made by the step's calls
+                             (bh_source*, this transport) since the last
+                             stats report; each synchronises the device.
+                             0 for steps within bh_options.reserve_photons */
 } bh_step_stats;
 
 /* A photon in the reference's 120-byte AOS layout (src/photon.h:172-181). */

@@ -54,6 +54,8 @@ typedef struct bh_driver_report {
   int32_t lattice;       /* 1: tracked on the lattice records               */
   int32_t tally;         /* tally form (bh_step_stats.tally)                */
   double replay_ms;      /* FP32 tracking: FP64 replay part of transport_ms */
+  uint64_t n_allocs;     /* device allocations in the step's device phase
+                            (bh_step_stats.n_allocs)                       */
 } bh_driver_report;
 
 typedef struct bh_driver bh_driver;

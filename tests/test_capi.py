@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert branson_amd.lib().bh_abi_version() == 3
+    assert branson_amd.lib().bh_abi_version() == 4
 
 
 def test_null_arguments_are_errors_not_crashes():

@@ -92,3 +92,24 @@ def test_long_run_keeps_conservation_as_the_census_grows():
         prev_census = r["census_size"]
     sizes = [r["census_size"] for r in reps]
     assert sizes[4] > 2 * sizes[0] and abs(sizes[-1] - sizes[-2]) < 0.05 * sizes[-1]
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(precision=capi.PRECISION_FP32_TRACK),
+                                dict(algorithm=capi.ALGO_EVENT), dict(step_graph=True),
+                                dict(device_physics=True)],
+                         ids=["fp64", "fp32", "event", "graph", "device_physics"])
+def test_steps_within_the_reservation_allocate_nothing(kw):
+    """Device allocations synchronise the device; with the photon stores and
+    every buffer sized from them reserved at creation, no step's device phase
+    allocates.  Without a reservation the first step does (the counter
+    counts)."""
+    deck = os.path.join(ROOT, "branson_amd", "decks", "cube_40_throughput.xml")
+    kw = dict(dict(algorithm=capi.ALGO_PERSISTENT), **kw)
+    with capi.Driver(deck, n_photons=1_000_000, t_stop=0.004, reserve_photons=5_000_000,
+                     **kw) as d:
+        reps = d.run()
+    print(kw, [r["n_allocs"] for r in reps], flush=True)
+    assert len(reps) == 4
+    assert [r["n_allocs"] for r in reps] == [0, 0, 0, 0]
+    with capi.Driver(deck, n_photons=1_000_000, t_stop=0.001, **kw) as d:
+        assert d.step()["n_allocs"] > 0
