@@ -568,8 +568,12 @@ int launch_replay(bh_handle* h, const DevCells& cells, const PhotonSoA& W, const
   // the replay's share of the transport time (not inside a captured graph)
   h->replay_timed = P == nullptr;
   if (h->replay_timed) HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
-  k_transport_replay<T><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(W, co, sd, cells, rl,
-                                                                     s.ctr_hi, t, s.ctrs, P);
+  if (sd.on)
+    k_transport_replay<T, true><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(
+        W, co, sd, cells, rl, s.ctr_hi, t, s.ctrs, P);
+  else
+    k_transport_replay<T, false><<<(unsigned)h->n_cu * 4, 256, shm, h->stream>>>(
+        W, co, sd, cells, rl, s.ctr_hi, t, s.ctrs, P);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
 }

@@ -453,12 +453,15 @@ __global__ __launch_bounds__(256) void k_source_fill(
   }
 }
 
-// Photon p of the step's source, from its draws u[0..6] of RNG(seed,
-// stream) at counters 0..6 (a face-source photon uses six): the sampling of
+// Photon p of the step's source, from its draws of RNG(seed, stream) at
+// counters 0, 1, ... (seven for an emission photon, six for a face-source or
+// initial-census one), taken in order from draw(): the sampling of
 // set_initial_census_photon / set_emission_photon /
 // set_boundary_source_photon (src/source.h:27-81) in the reference's order.
-// Shared by the source kernel and the transport kernels that make new
-// photons where they are transported.
+// Shared by the source kernel (draw() runs the generator) and the transport
+// kernels that make new photons where they are transported (draw() reads the
+// wave's draws); a draw functor rather than an array of draws, which a
+// branch-dependent index would put in scratch memory.
 struct SourcePhoton {
   double x, y, z, u, v, w, E, life;
   uint64_t ctr, stream;
@@ -478,60 +481,64 @@ __device__ __forceinline__ uint64_t source_stream(const SourceInst& sp,
   return sp.stream_base + (p - off[sp.first_cell]);
 }
 
+template <class Draw>
 __device__ __forceinline__ SourcePhoton make_source_photon(
     const DevCells& cells, const double* __restrict__ emission_E,
     const double* __restrict__ source_E, const int8_t* __restrict__ face_of,
     const uint64_t* __restrict__ off, uint64_t p, uint32_t c, const SourceInst& sp, double dt,
-    int mode, const double* u) {
+    int mode, Draw&& draw) {
   const uint64_t local = p - off[c];
   const uint64_t n_user = sp.n_user;
   const double total_E = sp.total_E;
   const CellGeo& g = cells.geo[c];
-  const double nd[6] = {g.nodes[0], g.nodes[1], g.nodes[2], g.nodes[3], g.nodes[4], g.nodes[5]};
+  const double n0 = g.nodes[0], n1 = g.nodes[1], n2 = g.nodes[2], n3 = g.nodes[3],
+               n4 = g.nodes[4], n5 = g.nodes[5];
   SourcePhoton o;
-  int k = 0;
   const uint32_t n_em = photons_for(n_user, emission_E[c], total_E);
   if (mode == 1 || local < n_em) {  // set_initial_census_photon / set_emission_photon
     o.E = emission_E[c] / (double)n_em;
-    o.x = nd[0] + u[0] * (nd[1] - nd[0]);  // get_uniform_position_in_cell
-    o.y = nd[2] + u[1] * (nd[3] - nd[2]);
-    o.z = nd[4] + u[2] * (nd[5] - nd[4]);
-    const double mu = u[3] * 2.0 - 1.0;  // get_uniform_angle
-    const double phi = u[4] * 2.0 * kPi;
+    o.x = n0 + draw() * (n1 - n0);  // get_uniform_position_in_cell
+    o.y = n2 + draw() * (n3 - n2);
+    o.z = n4 + draw() * (n5 - n4);
+    const double mu = draw() * 2.0 - 1.0;  // get_uniform_angle
+    const double phi = draw() * 2.0 * kPi;
     const double sin_theta = sqrt(1.0 - mu * mu);
     o.u = sin_theta * cos(phi);
     o.v = sin_theta * sin(phi);
     o.w = mu;
-    k = 5;
     if (mode == 1) {
       o.life = kC * dt;
       o.stype = 0;
+      o.ctr = 6;
     } else {
-      o.life = u[k++] * kC * dt;
+      o.life = draw() * kC * dt;
       o.stype = 2;
+      o.ctr = 7;
     }
   } else {  // set_boundary_source_photon
     const uint32_t n_src = photons_for(n_user, source_E[c], total_E);
     o.E = source_E[c] / (double)n_src;
     const int face = face_of[c];
     // get_uniform_position_on_face (src/sampling_functions.h:32-53)
+    const double a = draw();
+    const double b = draw();
     if (face == 0 || face == 1) {
-      o.x = (face == 0) ? nd[0] : nd[1];
-      o.y = nd[2] + u[0] * (nd[3] - nd[2]);
-      o.z = nd[4] + u[1] * (nd[5] - nd[4]);
+      o.x = (face == 0) ? n0 : n1;
+      o.y = n2 + a * (n3 - n2);
+      o.z = n4 + b * (n5 - n4);
     } else if (face == 2 || face == 3) {
-      o.x = nd[0] + u[0] * (nd[1] - nd[0]);
-      o.y = (face == 2) ? nd[2] : nd[3];
-      o.z = nd[4] + u[1] * (nd[5] - nd[4]);
+      o.x = n0 + a * (n1 - n0);
+      o.y = (face == 2) ? n2 : n3;
+      o.z = n4 + b * (n5 - n4);
     } else {
-      o.x = nd[0] + u[0] * (nd[1] - nd[0]);
-      o.y = nd[2] + u[1] * (nd[3] - nd[2]);
-      o.z = (face == 4) ? nd[4] : nd[5];
+      o.x = n0 + a * (n1 - n0);
+      o.y = n2 + b * (n3 - n2);
+      o.z = (face == 4) ? n4 : n5;
     }
     // get_source_angle_on_face (src/sampling_functions.h:94-122): inward
     // sign on x faces only, as the reference does
-    const double theta = acos(sqrt(u[2]));
-    const double phi = u[3] * 2.0 * kPi;
+    const double theta = acos(sqrt(draw()));
+    const double phi = draw() * 2.0 * kPi;
     const double sign = (face % 2) ? -1.0 : 1.0;
     const double ct = cos(theta), st = sin(theta);
     if (face == 0 || face == 1) {
@@ -547,12 +554,11 @@ __device__ __forceinline__ SourcePhoton make_source_photon(
       o.v = st * sin(phi);
       o.w = ct;
     }
-    k = 4;
-    o.life = u[k++] * kC * dt;
+    o.life = draw() * kC * dt;
     o.stype = 1;
+    o.ctr = 6;
   }
-  o.group = (uint32_t)floor(u[k++] * (double)cells.G);
-  o.ctr = (uint64_t)k;
+  o.group = (uint32_t)floor(draw() * (double)cells.G);
   o.stream = source_stream(sp, off, p);
   o.cell = c;
   return o;
@@ -578,13 +584,9 @@ __device__ __forceinline__ void source_one(const DevCells& cells,
                                            const SourceInst* __restrict__ inst, double dt,
                                            int mode, const PhotonSoA& W, uint64_t out_base) {
   const SourceInst sp = source_record(cells, c, one, inst);
-  double u[kSourceDraws];
   Rng rng{0ull, source_stream(sp, off, p), sp.ctr_hi};
-  const int nd = source_draw_count(emission_E, off, p, c, sp, mode);
-#pragma unroll
-  for (int k = 0; k < kSourceDraws; ++k) u[k] = (k < nd) ? rng.next() : 0.0;
   const SourcePhoton o = make_source_photon(cells, emission_E, source_E, face_of, off, p, c, sp,
-                                            dt, mode, u);
+                                            dt, mode, [&]() { return rng.next(); });
   const uint64_t i = out_base + p;
   // an initial census (mode 1) is in the reference's order by p; new photons
   // carry p too

@@ -503,13 +503,9 @@ __device__ __forceinline__ void gen_photon_lane(const DevCells& cells, const Sou
                                                 Live<R>& L) {
   const uint32_t c = hinted_cell(sd, cells.n_cells, n_new, p);
   const SourceInst sp = source_record(cells, c, sd.one, sd.inst);
-  const int nd = source_draw_count(sd.em, sd.off, p, c, sp, 0);
   Rng rng{0ull, source_stream(sp, sd.off, p), sp.ctr_hi};
-  double u[kSourceDraws];
-#pragma unroll
-  for (int k = 0; k < kSourceDraws; ++k) u[k] = (k < nd) ? rng.next() : 0.0;
   const SourcePhoton o = make_source_photon(cells, sd.em, sd.src, sd.face, sd.off, p, c, sp, dt,
-                                            0, u);
+                                            0, [&]() { return rng.next(); });
   live_from_source<R, kLat>(cells, o, sp.ctr_hi, L);
 }
 
@@ -1165,8 +1161,11 @@ struct CensusLocal {
 // one atomic, when it fills), with the reference's census life, and their
 // transport index is marked for the census order.  kDense: the wave takes
 // exactly the slots its census photons fill, one atomic per call (kernels
-// launched many times per step, which must leave no holes).
-template <typename R, bool kLat, bool kDense = false>
+// launched many times per step, which must leave no holes).  kLiveType:
+// the source type comes from the lane (kernels that make new photons);
+// otherwise from the photon's slot of W, so that the lane does not carry it
+// through the history.
+template <typename R, bool kLat, bool kDense = false, bool kLiveType = false>
 __device__ __forceinline__ void census_append(bool ended, const DevCells& cells, uint64_t i,
                                               const Live<R>& L, const CensusRun& cr,
                                               CensusLocal& loc) {
@@ -1209,7 +1208,7 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
       co.C.life[s] = cr.life;
       co.C.E0[s] = L.E0;
       co.C.stream[s] = L.rng.stream;
-      co.C.stype[s] = L.stype;
+      co.C.stype[s] = kLiveType ? L.stype : W.stype[i];
       co.C.key[s] = (uint32_t)t;
       // the census energy by transport index marks the photon (-0.0 for a
       // zero weight: nonzero bits, adds nothing); every key is set by its
@@ -1318,7 +1317,11 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
                                             uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
                                             PersistentQueue* q, int refill, uint32_t chunk,
                                             ReplayList rl, const StepParams* P) {
-  if (!args_ok(W, co, sd, ctrs)) return;
+  if constexpr (kGen) {
+    if (!args_ok(W, co, sd, ctrs)) return;
+  } else {
+    if (!args_ok(W, co, ctrs)) return;
+  }
   const uint64_t kChunk = chunk;
   if (P) n = P->n;  // a step captured as a graph
   const CensusRun cr = census_run(co, P);
@@ -1385,11 +1388,10 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
         WaveDraws gd;
         wave_gen_draws(gen, nd, stream, hi, gd);
         if (gen) {
-          double u[kSourceDraws];
-#pragma unroll
-          for (int k = 0; k < kSourceDraws; ++k) u[k] = (k < nd) ? gd.res[gd.next + k] : 0.0;
+          uint32_t j = gd.next;
           const SourcePhoton o = make_source_photon(cells, s.em, s.src, s.face, s.off, p, c, sp,
-                                                    P ? P->dt : s.dt, 0, u);
+                                                    P ? P->dt : s.dt, 0,
+                                                    [&]() { return gd.res[j++]; });
           live_from_source<R, kLat>(cells, o, hi, L);
         }
         gen = false;
@@ -1411,7 +1413,7 @@ BH_TRACK_KERNEL void k_transport_persistent(const PhotonSoA W, const CensusOut c
     }
     stuck += (unsigned long long)__popcll(__ballot(ended && L.desc == kStuck));
     if constexpr (sizeof(R) == 4) replay_append(replay, idx, L.first_seg, L.E_handoff, rl, ctrs);
-    if (co.on) census_append<R, kLat>(ended && !replay, cells, idx, L, cr, loc);
+    if (co.on) census_append<R, kLat, false, kGen>(ended && !replay, cells, idx, L, cr, loc);
     flushes += (unsigned long long)__popcll(__ballot(fo.has));
     tally.push(fo);
   }
@@ -1587,14 +1589,21 @@ __global__ __launch_bounds__(256) void k_ev_scatter(PhotonSoA W, DevCells cells,
 // (the cutoff test included) still uses the unscaled FP64 weight.  The list
 // length is read on the device, so no host round trip separates the two
 // launches.  Loop bounds are block-uniform.
-template <class Tally>
+// kGen: the new photons of the step were made in the transport kernel (the
+// source description is on), so new ones are made again here; a separate
+// instance keeps that path's registers out of the plain replay.
+template <class Tally, bool kGen>
 BH_TRACK_KERNEL void k_transport_replay(const PhotonSoA W, const CensusOut co,
                                         const SourceDesc sd, DevCells cells, ReplayList rl,
                                         uint64_t ctr_hi, Tally tally, DevCounters* ctrs,
                                         const StepParams* P) {
-  if (!args_ok(W, co, sd, ctrs)) return;
+  if constexpr (kGen) {
+    if (!args_ok(W, co, sd, ctrs)) return;
+  } else {
+    if (!args_ok(W, co, ctrs)) return;
+  }
   const CensusRun cr = census_run(co, P);
-  const uint64_t first_gen = sd.on ? cr.n_cen : ~0ull;
+  const uint64_t first_gen = (kGen && sd.on) ? cr.n_cen : ~0ull;
   CensusLocal loc;
   bind_lds(tally, cells.n_cells);
   tally.init(cells.n_cells);
@@ -1608,7 +1617,7 @@ BH_TRACK_KERNEL void k_transport_replay(const PhotonSoA W, const CensusOut co,
     Live<double> L;
     if (active) {
       i = rl.idx[j];
-      if (i >= first_gen)  // a new photon: made again from its draws
+      if (kGen && i >= first_gen)  // a new photon: made again from its draws
         gen_photon_lane<double, false>(cells, arg_source(), cr.n_new, P ? P->dt : sd.dt,
                                        i - first_gen, L);
       else
@@ -1645,7 +1654,7 @@ BH_TRACK_KERNEL void k_transport_replay(const PhotonSoA W, const CensusOut co,
           ended = true;
         }
       }
-      if (co.on) census_append<double, false>(ended, cells, i, L, cr, loc);
+      if (co.on) census_append<double, false, false, kGen>(ended, cells, i, L, cr, loc);
       flushes += fo.has;
       tally.push(fo);
     }
