@@ -254,6 +254,7 @@ CUBE_WORKLOAD = ("cube_40_throughput: plain gray 3D cube, 40x40x40 cells of 0.05
                  "photons/step, seed 1234")
 HOHLRAUM_PHOTONS = 50_000_000
 TALLY_BAR = 1e-6  # per-cell absorbed-energy error the chosen form must meet
+CUBE_TRAFFIC = os.path.join(ROOT, "profiles", "transport_persistent_fp64_cube40_traffic.md")
 HOHLRAUM_TRAFFIC = os.path.join(ROOT, "profiles",
                                 "transport_persistent_fp64_hohlraum64_traffic.md")
 
@@ -359,7 +360,9 @@ def _bench_headline(a, capi, cube=True):
             "kernel_ms_per_step": _kernel_split(c),
             "transport_kernel_roofline": {"achieved_GBps": cach, "frac": cach / HBM_PEAK_GBPS,
                                           "kernel_ms_mean": cms,
-                                          "algorithmic_bytes_per_launch": cb},
+                                          "algorithmic_bytes_per_launch": cb,
+                                          "launch_histories_segments_flushes": c["launches"],
+                                          "launch_kernel_ms": c["k_ms"]},
             "event_form": {"histories_per_s": ce["hist"] / ce["dev_s"],
                            "ms_per_step": 1e3 * ce["dev_s"] / a.steps,
                            "kernel_ms_per_step": _kernel_split(ce)}}
@@ -516,10 +519,14 @@ def _bench_cube(a, capi):
                        "step1_tally_error_vs_fp64": _tally_error(plain["step1"], ch["step1"]),
                        "replay_fraction": ch["replayed"] / max(ch["hist"], 1)},
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                  "kernel": "k_transport_persistent (plain fp64, cube)",
+                  "frac": achieved / HBM_PEAK_GBPS,
+                  "traffic": _traffic_from_profile(CUBE_TRAFFIC, a.steps, a.warmup),
+                  "kernel": "k_transport_persistent<GlobalTally<false>, double, true> "
+                            "(plain fp64, cube)",
                   "kernel_ms_mean": kms, "algorithmic_bytes_per_launch": bytes_launch,
-                  "launches": len(plain["k_ms"]), "copy_probe_GBps": copy_gbps},
+                  "launches": len(plain["k_ms"]),
+                  "launch_histories_segments_flushes": plain["launches"],
+                  "launch_kernel_ms": plain["k_ms"], "copy_probe_GBps": copy_gbps},
         cpu_baseline=None if a.no_cpu else _cpu_baseline(
             DECK, 1, N_PHOTONS, 2, "steps 1-2 of the same 40^3 cube deck at 1e7 photons/step"))
 
