@@ -220,6 +220,14 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
 }
 
+// Sum over the 64 lanes of a wave, in a fixed butterfly order; every lane
+// gets the sum.  All lanes at the same program point.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Generic exclusive scan of uint64 counts: reduce per tile, scan the tile
 // sums in one block, then scan each tile with its offset.

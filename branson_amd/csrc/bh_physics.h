@@ -377,11 +377,6 @@ struct InstReport {  // bh_instance_report, field for field
   uint64_t n_new;
 };
 
-__device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
-  return v;
-}
 
 // After the physics of the step: energies, material energy and total_E per
 // instance; total_E goes into the instance's source records (census, new).

@@ -1175,9 +1175,24 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
   // pointer-selected increment that puts them in scratch memory.
   const bool ex = ended && L.desc == kExit, kl = ended && L.desc == kKilled;
   loc.exit_E += ex ? (double)L.E : 0.0;
-  if (ex && cells.cell_inst) {  // several problems: each instance's exit energy
+  if (cells.cell_inst) {  // several problems: each instance's exit energy
     const CensusOut co = arg_census();
-    if (co.inst_exit) atomicAdd(&co.inst_exit[cells.cell_inst[L.cell]], (double)L.E);
+    uint64_t pending = co.inst_exit ? __ballot(ex) : 0ull;
+    if (pending) {
+      // one atomic per instance among this iteration's exits (a wave's
+      // photons mostly share an instance), not one per photon: per-photon
+      // atomics on a few hundred addresses serialise
+      const uint32_t inst = ex ? cells.cell_inst[L.cell] : 0u;
+      const int lane = (int)(threadIdx.x & (kWave - 1));
+      while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t li = (uint32_t)__shfl((int)inst, leader, kWave);
+        const bool mine = ex && inst == li;
+        const double v = wave_sum_f64(mine ? (double)L.E : 0.0);
+        if (lane == leader) atomicAdd(&co.inst_exit[li], v);
+        pending &= ~__ballot(mine);
+      }
+    }
   }
   loc.n_exit += (uint32_t)__popcll(__ballot(ex));
   loc.n_killed += (uint32_t)__popcll(__ballot(kl));
