@@ -415,8 +415,12 @@ __global__ __launch_bounds__(kWave) void k_inst_sums(const double* __restrict__ 
 }
 
 // Census energy per instance: the photons of W[0, *n) (or [0, n)) summed
-// into out[instance] (post = 1: post_census_E, else pre_census_E).  Atomic
-// adds, each wave's adds combined when all its photons are in one instance.
+// into out[instance] (post = 1: post_census_E, else pre_census_E).  Each
+// wave walks a contiguous range of the census, which is in census order, so
+// its photons stay in one instance for long runs: the wave keeps a per-lane
+// partial for the current instance and adds it with one atomic when the
+// instance changes; an iteration whose photons span instances adds lane by
+// lane.
 __global__ __launch_bounds__(256) void k_inst_census(const uint32_t* __restrict__ cell,
                                                      const double* __restrict__ E, uint64_t n,
                                                      const unsigned long long* __restrict__ n_ptr,
@@ -425,22 +429,40 @@ __global__ __launch_bounds__(256) void k_inst_census(const uint32_t* __restrict_
                                                      InstReport* __restrict__ rep, int post) {
   if (n_ptr) n = *n_ptr < n ? *n_ptr : n;
   if (n_ptr64) n = *n_ptr64 < n ? *n_ptr64 : n;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += stride) {
-    const uint64_t i = b + threadIdx.x;
-    const bool in = i < n;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  // this wave's range, in whole iterations of 64 photons
+  const uint64_t per = (n + waves * kWave - 1) / (waves * kWave) * kWave;
+  const uint64_t lo = wave * per;
+  const uint64_t hi = lo + per < n ? lo + per : n;
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  double* dst = post ? &rep[0].post_census_E : &rep[0].pre_census_E;
+  const size_t stride_d = sizeof(InstReport) / sizeof(double);
+  uint32_t cur = 0xFFFFFFFFu;  // wave-uniform: the instance `part` belongs to
+  double part = 0.0;
+  for (uint64_t b = lo; b < hi; b += kWave) {
+    const uint64_t i = b + (uint64_t)lane;
+    const bool in = i < hi;
     const uint32_t k = in ? cell_inst[cell[i]] : 0u;
     const double e = in ? E[i] : 0.0;
     const uint32_t k0 = (uint32_t)__shfl((int)k, 0, kWave);
     const bool same = __ballot(in && k != k0) == 0;
-    double* dst = post ? &rep[0].post_census_E : &rep[0].pre_census_E;
-    const size_t stride_d = sizeof(InstReport) / sizeof(double);
-    if (same) {
-      const double w = wave_sum_f64(e);
-      if ((threadIdx.x & (kWave - 1)) == 0 && w != 0.0) atomicAdd(dst + k0 * stride_d, w);
+    if (same && k0 == cur) {
+      part += e;
+    } else if (same) {
+      if (cur != 0xFFFFFFFFu) {
+        const double w = wave_sum_f64(part);
+        if (lane == 0 && w != 0.0) atomicAdd(dst + cur * stride_d, w);
+      }
+      cur = k0;
+      part = e;
     } else if (in) {
       atomicAdd(dst + k * stride_d, e);
     }
+  }
+  if (cur != 0xFFFFFFFFu) {
+    const double w = wave_sum_f64(part);
+    if (lane == 0 && w != 0.0) atomicAdd(dst + cur * stride_d, w);
   }
 }
 
