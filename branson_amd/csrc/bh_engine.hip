@@ -150,7 +150,10 @@ struct bh_handle {
   DArr<uint8_t> comb_keep, comb_tmp;
   DArr<double> comb_cellE;
   DArr<unsigned long long> comb_ctr;
-  DArr<uint64_t> cflag;       // census energies by transport index (nonzero: census)
+  DArr<uint64_t> cflag;       // keep flags of the comb
+  DArr<uint64_t> cmark;       // census marks by transport index, one byte each
+  DArr<uint64_t> mwords, mcounts;  // the marks 64 to a word: bit masks and counts
+  DArr<double> erank;         // census energies by rank (census order)
   DArr<HoleRec> hrec;         // census slots a wave took and left unused
   DArr<uint64_t> hole_len, hole_off, tail_live, tail_rank;  // hole fill
   CensusAcc* d_cacc = nullptr;  // census totals of a census-writing launch
@@ -779,6 +782,20 @@ CensusSizes census_sizes(const bh_handle* h, uint64_t n) {
   return z;
 }
 
+// The census marks of launches of up to pc photons: one byte per photon
+// and a whole word (64) of zero marks past the last.
+inline uint64_t mark_words(uint64_t pc) { return div_up(std::max<uint64_t>(pc, 1), 64) + 1; }
+
+int ensure_marks(bh_handle* h, uint64_t pc) {
+  const uint64_t mw = mark_words(pc);
+  TRY(ensure(h, h->cmark, mw * 8));
+  TRY(ensure(h, h->mwords, mw));
+  TRY(ensure(h, h->mcounts, mw));
+  TRY(ensure(h, h->off, mw));
+  TRY(ensure(h, h->erank, std::max<uint64_t>(pc, 1)));
+  return BH_OK;
+}
+
 // Buffers and the output record of a census-writing launch over the n
 // photons of W = [census | new]: C has room for all of them and the holes
 // the waves leave, the flags by transport index are cleared, the totals
@@ -792,19 +809,19 @@ int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co) {
   const CensusSizes z = census_sizes(h, cap);
   TRY(grow_store(h, h->C, z.slots, 0));
   const uint64_t pc = photon_cap(h, cap);
-  TRY(ensure(h, h->cflag, pc));
-  TRY(ensure(h, h->off, pc));
+  TRY(ensure_marks(h, pc));
   TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(pc, kScanTile), 1)));
   TRY(ensure(h, h->tile_E, std::max<uint64_t>(div_up(pc, kScanTile), 1)));
   TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
   TRY(ensure(h, h->hrec, z.rec_cap));
-  k_fill_u64<<<grid_for(std::max<uint64_t>(cap, 1), 256), 256, 0, h->stream>>>(
-      reinterpret_cast<unsigned long long*>(h->cflag.p), std::max<uint64_t>(cap, 1));
+  const uint64_t mw = mark_words(cap);  // the marks of this launch and a zero word
+  k_fill_u64<<<grid_for(mw * 8, 256), 256, 0, h->stream>>>(
+      reinterpret_cast<unsigned long long*>(h->cmark.p), mw * 8);
   k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_cacc),
                                       sizeof(CensusAcc) / 8);
   HIPCHK(h, hipGetLastError());
-  co = CensusOut{h->C.soa, h->cflag.p, h->d_cacc, h->hrec.p, kC * next_dt,
-                 h->n_cen,  h->n_new,   h->C.cap,  z.rec_cap,  1};
+  co = CensusOut{h->C.soa, reinterpret_cast<uint8_t*>(h->cmark.p), h->d_cacc, h->hrec.p,
+                 kC * next_dt, h->n_cen, h->n_new, h->C.cap, z.rec_cap, 1};
   return BH_OK;
 }
 
@@ -831,12 +848,22 @@ int census_finish(bh_handle* h, uint64_t cap, const uint64_t* n_ptr) {
                                              h->tail_rank.p, h->hrec.p, h->hole_off.p, z.rec_cap);
   HIPCHK(h, hipGetLastError());
   const uint64_t n = std::max<uint64_t>(cap, 1);
-  TRY(ensure(h, h->off, n));
-  // ranks, and the census energy summed in census order on the way
-  TRY(scan_u64(h, h->cflag.p, n, h->off.p, 2, n_ptr, true, &h->d_ctot->census_E));
+  const uint64_t nw = div_up(n, 64);
+  TRY(ensure_marks(h, n));
+  // ranks: the marks 64 to a word, each word's first rank by a scan of
+  // their counts, then each census photon's rank from its word; the census
+  // energy summed in census order (by rank, a fixed split of the count)
+  k_mark_words<<<std::min<unsigned>(grid_for(nw, 256), 4096), 256, 0, h->stream>>>(
+      reinterpret_cast<const uint8_t*>(h->cmark.p), n, n_ptr, h->mwords.p, h->mcounts.p, acc);
+  TRY(scan_u64(h, h->mcounts.p, nw, h->off.p, 2,
+               reinterpret_cast<const uint64_t*>(&acc->n_words)));
   const unsigned long long* cnt = &acc->n_census;
   k_census_rekey<<<std::min<unsigned>(grid_for(n, 256), 8192), 256, 0, h->stream>>>(
-      h->C.soa.key, h->off.p, cnt, n);
+      h->C.soa.key, h->C.soa.E, h->mwords.p, h->off.p, cnt, n, h->erank.p);
+  k_rank_energy_partials<<<kSumMaxBlocks, kSumNT, 0, h->stream>>>(h->erank.p, cnt,
+                                                                  h->sum_part.p);
+  k_final_sums<1><<<1, kSumNT, 0, h->stream>>>(h->sum_part.p, kSumMaxBlocks,
+                                              &h->d_ctot->census_E);
   k_census_pack<<<1, 1, 0, h->stream>>>(acc, h->d_ctot);
   HIPCHK(h, hipGetLastError());
   return BH_OK;
@@ -897,8 +924,7 @@ int reserve_buffers(bh_handle* h, uint64_t n) {
   const uint64_t pc = photon_cap(h, n);
   const CensusSizes z = census_sizes(h, pc);
   const uint64_t tiles = std::max<uint64_t>(div_up(pc, kScanTile), 1);
-  TRY(ensure(h, h->cflag, pc));
-  TRY(ensure(h, h->off, pc));
+  TRY(ensure_marks(h, pc));
   TRY(ensure(h, h->tiles, tiles));
   TRY(ensure(h, h->tile_E, tiles));
   TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
@@ -1042,6 +1068,10 @@ void bh_destroy(bh_handle* h) {
   release(h->comb_cellE);
   release(h->comb_ctr);
   release(h->cflag);
+  release(h->cmark);
+  release(h->mwords);
+  release(h->mcounts);
+  release(h->erank);
   release(h->tile_E);
   release(h->hrec);
   release(h->hole_len);
@@ -1497,7 +1527,8 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
           u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
           u(h->h_tally),     (uintptr_t)h->n_cells, (uintptr_t)h->G,
           u(h->rec.p),       u(h->axes.p),    (uintptr_t)h->lattice,
-          u(h->cflag.p),     u(h->d_cacc),    u(h->sum_part.p),
+          u(h->cmark.p),     u(h->mwords.p),  u(h->mcounts.p),      u(h->erank.p),
+          u(h->d_cacc),      u(h->sum_part.p),
           u(h->hrec.p),      u(h->hole_len.p), u(h->hole_off.p),   u(h->tail_live.p),
           u(h->tail_rank.p), u(h->tile_E.p),
           (uintptr_t)h->gray_sigma_a, (uintptr_t)use_hash_tally(h), (uintptr_t)h->ctr_hi};
@@ -1579,10 +1610,9 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
   const CensusSizes z = census_sizes(h, cap);
   const uint64_t pc = photon_cap(h, cap);  // as census_prepare sizes them
   TRY(ensure(h, h->counts, nc));
-  TRY(ensure(h, h->off, std::max<uint64_t>(nc, pc)));
   TRY(ensure(h, h->soff, nc));
   TRY(ensure(h, h->src_hint, div_up(std::max<uint64_t>(cap, 1), kHintStride) + 1));
-  TRY(ensure(h, h->cflag, pc));
+  TRY(ensure_marks(h, pc));
   TRY(ensure(h, h->hrec, z.rec_cap));
   TRY(ensure(h, h->hole_len, z.rec_cap));
   TRY(ensure(h, h->hole_off, z.rec_cap));
@@ -2606,10 +2636,9 @@ int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uin
   const uint64_t pc = photon_cap(h, cap);
   const uint32_t nb = resident_blocks(h);
   TRY(ensure(h, h->counts, nc));
-  TRY(ensure(h, h->off, std::max<uint64_t>(nc, pc)));
   TRY(ensure(h, h->soff, nc));
   TRY(ensure(h, h->src_hint, div_up(std::max<uint64_t>(cap, 1), kHintStride) + 1));
-  TRY(ensure(h, h->cflag, pc));
+  TRY(ensure_marks(h, pc));
   TRY(ensure(h, h->hrec, z.rec_cap));
   TRY(ensure(h, h->hole_len, z.rec_cap));
   TRY(ensure(h, h->hole_off, z.rec_cap));

@@ -57,12 +57,13 @@ struct CensusAcc {  // zeroed before each launch
   unsigned long long n_exit, n_killed, n_other;
   unsigned long long overflow;    // a capacity bound was hit (never, by sizing)
   double exit_E;
-  double pad[7];
+  unsigned long long n_words;     // census marks' words, set after the launch
+  double pad[6];
 };
 
 struct CensusOut {
   PhotonSoA C;
-  uint64_t* flag;   // [n] census energy bits at each census photon's transport index
+  uint8_t* mark;    // [n] 1 at each census photon's transport index, else 0
   CensusAcc* acc;
   HoleRec* rec;     // [rec_cap]: one per wave that took a chunk
   double life;      // c * next_dt given to census photons
@@ -1033,13 +1034,59 @@ __global__ void k_census_perm(const uint32_t* __restrict__ key, uint64_t n,
 // index t; rank = exclusive prefix of flag over t gives its place in the
 // reference's census order.  count = survivors (on the device); rank holds
 // cap entries.
-__global__ void k_census_rekey(uint32_t* __restrict__ key, const uint64_t* __restrict__ rank,
-                               const unsigned long long* __restrict__ count, uint64_t cap) {
+// Census marks by transport index, 64 to a word: each word's bit mask and
+// its count (their exclusive scan gives each word's first rank), and the
+// word count into acc->n_words.  The marks are zero up to a whole word past
+// the launch's photons.
+__global__ void k_mark_words(const uint8_t* __restrict__ mark, uint64_t cap,
+                             const uint64_t* __restrict__ n_ptr, uint64_t* __restrict__ words,
+                             uint64_t* __restrict__ counts, CensusAcc* __restrict__ acc) {
+  uint64_t n = cap;
+  if (n_ptr) n = *n_ptr < cap ? *n_ptr : cap;
+  const uint64_t nw = (n + 63) / 64;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (first == 0) acc->n_words = nw;
+  for (uint64_t w = first; w < nw; w += stride) {
+    const uint4* q = reinterpret_cast<const uint4*>(mark + w * 64);
+    uint64_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint4 v = q[k];
+      const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {  // four 0/1 bytes to four bits
+        const uint32_t b = (x[j] & 1u) | ((x[j] >> 7) & 2u) | ((x[j] >> 14) & 4u) |
+                           ((x[j] >> 21) & 8u);
+        m |= (uint64_t)b << (16 * k + 4 * j);
+      }
+    }
+    words[w] = m;
+    counts[w] = (uint64_t)__popcll(m);
+  }
+}
+
+// Each census photon's key, its transport index t, becomes its rank in the
+// reference's census order: the census photons with smaller t, counted from
+// the mark words.  Its energy goes to position rank of E_rank, so that the
+// census energy can be summed in census order.
+__global__ void k_census_rekey(uint32_t* __restrict__ key, const double* __restrict__ E,
+                               const uint64_t* __restrict__ words,
+                               const uint64_t* __restrict__ word_rank,
+                               const unsigned long long* __restrict__ count, uint64_t cap,
+                               double* __restrict__ E_rank) {
   const uint64_t n = *count;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
     const uint32_t t = key[s];
-    key[s] = t < cap ? (uint32_t)rank[t] : 0xFFFFFFFFu;
+    if (t < cap) {
+      const uint64_t w = t >> 6, below = (1ull << (t & 63u)) - 1ull;
+      const uint64_t r = word_rank[w] + (uint64_t)__popcll(words[w] & below);
+      key[s] = (uint32_t)r;
+      E_rank[r] = E[s];
+    } else {
+      key[s] = 0xFFFFFFFFu;
+    }
   }
 }
 

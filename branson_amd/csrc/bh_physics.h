@@ -132,6 +132,20 @@ __global__ __launch_bounds__(kSumNT) void k_census_partials(const double* __rest
   if (threadIdx.x == 0) part[blockIdx.x * 8] = v[0];
 }
 
+// Deterministic block partials of E[0, *n) (a count read on the device),
+// one per block of a fixed grid of kSumMaxBlocks: the split depends on n
+// only.  k_final_sums<1> ends the sum.
+__global__ __launch_bounds__(kSumNT) void k_rank_energy_partials(
+    const double* __restrict__ E, const unsigned long long* __restrict__ n_ptr,
+    double* __restrict__ part) {
+  double v[1] = {0.0};
+  uint32_t lo, hi;
+  sum_range((uint32_t)*n_ptr, lo, hi);
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kSumNT) v[0] += E[i];
+  block_sums<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x * 8] = v[0];
+}
+
 // One block: dst[k] = sum over the nb block partials of quantity k.
 template <int K>
 __global__ __launch_bounds__(kSumNT) void k_final_sums(const double* __restrict__ part,

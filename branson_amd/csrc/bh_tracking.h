@@ -1119,7 +1119,7 @@ __device__ __forceinline__ bool args_ok(const PhotonSoA& W, const CensusOut& co,
   const PhotonSoA w = arg_stores();
   const CensusOut c = arg_census();
   const bool ok = w.x == W.x && w.key == W.key && c.C.x == co.C.x && c.C.key == co.C.key &&
-                  c.flag == co.flag && c.acc == co.acc && c.rec == co.rec &&
+                  c.mark == co.mark && c.acc == co.acc && c.rec == co.rec &&
                   c.slots == co.slots && c.rec_cap == co.rec_cap && c.on == co.on;
   if (!ok && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctrs->bad_args, 1ull);
   return ok;
@@ -1225,11 +1225,9 @@ __device__ __forceinline__ void census_append(bool ended, const DevCells& cells,
       co.C.stream[s] = L.rng.stream;
       co.C.stype[s] = kLiveType ? L.stype : W.stype[i];
       co.C.key[s] = (uint32_t)t;
-      // the census energy by transport index marks the photon (-0.0 for a
-      // zero weight: nonzero bits, adds nothing); every key is set by its
-      // producer
-      if (t < cr.n_cen + cr.n_new)
-        co.flag[t] = (uint64_t)__double_as_longlong(L.E != 0.0 ? (double)L.E : -0.0);
+      // the photon's mark at its transport index gives its rank in the
+      // census order after the launch; every key is set by its producer
+      if (t < cr.n_cen + cr.n_new) co.mark[t] = 1;
     } else {
       atomicAdd(&co.acc->overflow, 1ull);
     }
