@@ -712,8 +712,7 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
     return (unsigned)std::min<uint64_t>((uint64_t)h->n_cu * bpc,
                                         std::max<uint64_t>(div_up(n, block), 1));
   };
-  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(s.queue),
-                                      sizeof(PersistentQueue) / 8);
+  k_queue_init<<<1, 64, 0, h->stream>>>(s.queue);
   return with_tally(h, lds, s, [&](auto t, size_t shm) -> int {
     using T = decltype(t);
     // one instantiation per (precision, records, in-kernel photon making)

@@ -51,10 +51,52 @@
 
 namespace bh {
 
+// The persistent kernel's work queue: chunk c of the launch's photons is
+// handed out by counter c % 8, so the waves claim through eight counters,
+// each on a 128-byte line of its own (atomics from all waves on one line
+// serialise), while the photons in flight still form one window moving
+// through the stores in order.  A wave starts at the counter of its XCD.
+constexpr uint32_t kQueueParts = 8;
 struct PersistentQueue {
-  unsigned long long next;
-  unsigned long long pad[7];
+  struct alignas(128) Counter {
+    unsigned long long next;  // chunks this counter handed out
+  } part[kQueueParts];
 };
+
+// This wave's XCD (hardware register XCC_ID), 0-7 on MI355X.
+__device__ __forceinline__ uint32_t xcc_id() {
+#if __HIP_DEVICE_COMPILE__
+  uint32_t v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return v & (kQueueParts - 1u);
+#else
+  return 0;
+#endif
+}
+
+// Before each persistent launch: the counters zeroed.
+__global__ void k_queue_init(PersistentQueue* q) {
+  if (threadIdx.x < kQueueParts) q->part[threadIdx.x].next = 0;
+}
+
+// Claims the next chunk of photon indices from the counter of the wave's
+// XCD, or from the others in turn once it is used up.  Returns the chunk's
+// first index and sets its end, or returns n when every counter is used up.
+__device__ __forceinline__ uint64_t claim_chunk(PersistentQueue* q, uint64_t n, uint64_t chunk,
+                                                uint64_t& end) {
+  const uint32_t xcc = xcc_id();
+#pragma unroll 1
+  for (uint32_t k = 0; k < kQueueParts; ++k) {
+    const uint32_t part = (xcc + k) & (kQueueParts - 1u);
+    const uint64_t c = atomicAdd(&q->part[part].next, 1ull);
+    const uint64_t base = (c * kQueueParts + part) * chunk;
+    if (base < n) {
+      end = base + chunk < n ? base + chunk : n;
+      return base;
+    }
+  }
+  return n;
+}
 
 template <typename R>
 struct Live {
@@ -1365,15 +1407,16 @@ void k_transport_persistent(const PhotonSoA W, const CensusOut co,
       // hand out indices from the wave's chunk, claiming chunks as needed
       while (n_idle > 0 && !drained) {
         if (chunk_next >= chunk_end) {
-          unsigned long long base = 0;
-          if (lane == 0) base = atomicAdd(&q->next, (unsigned long long)kChunk);
-          base = __shfl(base, 0, kWave);
+          uint64_t base = 0, end = 0;
+          if (lane == 0) base = claim_chunk(q, n, kChunk, end);
+          base = shfl_u64(base, 0);
+          end = shfl_u64(end, 0);
           if (base >= n) {
             drained = true;
             break;
           }
           chunk_next = base;
-          chunk_end = (base + kChunk < n) ? base + kChunk : n;
+          chunk_end = end;
         }
         const uint64_t avail = chunk_end - chunk_next;
         const uint64_t rank = lane_rank(idle);
