@@ -131,6 +131,8 @@ struct bh_handle {
   bool gray_sigma_a = false;     // one tally atomic per flush (see with_tally)
   bool uniform_sigma_a = false;  // sigma_a of every cell equal in all groups
   bool lattice = false;          // mesh is a lattice: CellRec + axis arrays
+  bool inst_lattice = false;     // every instance a lattice: CellRec + per-instance axes
+  int last_lattice = 0;          // records of the last launch (bh_step_stats.lattice)
   double src_neff_frac = 1.0;    // effective source cells / n_cells (tally choice)
   int last_tally = 0;            // tally form of the last launch (bh_step_stats.tally)
   uint64_t n_allocs = 0;         // device allocations of step calls since the last stats report
@@ -160,6 +162,7 @@ struct bh_handle {
   uint64_t serial_etot_steps = 0; // steps whose E_tot needed the serial sum
   uint32_t dims[3] = {0, 0, 0};
   DArr<CellRec> rec;
+  DArr<LatInst> lat;             // per-instance lattice shapes (inst_lattice)
   DArr<double> axes;             // x nodes, then y, then z
 
   // per-cell source energies
@@ -359,6 +362,10 @@ DevCells dev_cells(const bh_handle* h) {
     d.axis[1] = h->axes.p + h->dims[0] + 1;
     d.axis[2] = d.axis[1] + h->dims[1] + 1;
     for (int a = 0; a < 3; ++a) d.dims[a] = h->dims[a];
+  } else if (h->inst_lattice && inst) {
+    d.rec = h->rec.p;
+    d.axis[0] = h->axes.p;
+    d.lat = h->lat.p;
   }
   return d;
 }
@@ -417,7 +424,8 @@ int read_total(bh_handle* h, int slot, uint64_t* v) {
 // coordinates agree bit for bit with three axis arrays, ELEMENT faces whose
 // neighbour is the lattice neighbour, no PROCESSOR face, and opacities equal
 // in every group of a cell.  Every mesh built from a Branson deck is one.
-bool detect_lattice(const bh_cells* c, uint32_t dims[3], std::vector<double>& axes) {
+bool detect_lattice(const bh_cells* c, uint32_t dims[3], std::vector<double>& axes,
+                    uint32_t base = 0) {
   const uint32_t n = c->n_cells, G = c->n_groups;
   const double* nd = c->nodes;
   auto at = [&](uint32_t cell, int k) { return nd[(size_t)cell * 6 + k]; };
@@ -457,7 +465,7 @@ bool detect_lattice(const bh_cells* c, uint32_t dims[3], std::vector<double>& ax
             const bool hi = f & 1;
             if (hi ? idx[a] + 1 >= dim[a] : idx[a] == 0) return false;
             const uint64_t nb = hi ? cell + stride[a] : cell - stride[a];
-            if (c->next[(size_t)cell * 6 + f] != nb) return false;
+            if (c->next[(size_t)cell * 6 + f] != nb + base) return false;
           }
         }
         for (uint32_t g = 1; g < G; ++g)
@@ -469,6 +477,57 @@ bool detect_lattice(const bh_cells* c, uint32_t dims[3], std::vector<double>& ax
   dims[1] = ny;
   dims[2] = nz;
   return true;
+}
+
+// Many problems whose meshes are each a lattice (cells of instance k are
+// [first[k], first[k+1]), numbered x fastest within it, ELEMENT faces only
+// inside it): the 32-byte records with each cell's instance in CellRec::pad,
+// every instance's axes back to back, and the per-instance shapes.  Leaves
+// h->inst_lattice false (tracking on the general records) when any
+// instance is not a lattice.  c: the whole mesh in host memory.
+int build_inst_lattice(bh_handle* h, const bh_cells* c) {
+  h->inst_lattice = false;
+  if (h->opt.general_mesh != 0 || h->lattice || h->n_inst == 0) return BH_OK;
+  const uint32_t K = h->n_inst, G = c->n_groups;
+  std::vector<double> axes;
+  std::vector<LatInst> lat(K);
+  std::vector<CellRec> rec(c->n_cells);
+  for (uint32_t k = 0; k < K; ++k) {
+    const uint32_t f = h->inst_first[k], e = h->inst_first[k + 1];
+    bh_cells sub = *c;
+    sub.n_cells = e - f;
+    sub.nodes = c->nodes + (size_t)f * 6;
+    sub.bc = c->bc + (size_t)f * 6;
+    sub.next = c->next + (size_t)f * 6;
+    sub.sigma_a = c->sigma_a + (size_t)f * G;
+    sub.sigma_s = c->sigma_s + (size_t)f * G;
+    sub.fleck = c->fleck + f;
+    uint32_t dims[3];
+    std::vector<double> ax;
+    if (!detect_lattice(&sub, dims, ax, f)) return BH_OK;
+    const uint32_t o = (uint32_t)axes.size();
+    lat[k] = LatInst{f, dims[0], dims[1], dims[2], o, o + dims[0] + 1,
+                     o + dims[0] + dims[1] + 2, 0u};
+    axes.insert(axes.end(), ax.begin(), ax.end());
+    for (uint32_t i = f; i < e; ++i) {
+      uint32_t bits = 0;
+      for (int q = 0; q < 6; ++q) bits |= (uint32_t)c->bc[(size_t)i * 6 + q] << (3 * q);
+      rec[i] = CellRec{c->sigma_a[(size_t)i * G], c->sigma_s[(size_t)i * G], c->fleck[i], bits,
+                       k};
+    }
+  }
+  TRY(ensure(h, h->rec, c->n_cells));
+  TRY(ensure(h, h->axes, axes.size()));
+  TRY(ensure(h, h->lat, K));
+  HIPCHK(h, hipMemcpyAsync(h->rec.p, rec.data(), rec.size() * sizeof(CellRec),
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->axes.p, axes.data(), axes.size() * sizeof(double),
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->lat.p, lat.data(), K * sizeof(LatInst), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->inst_lattice = true;
+  return BH_OK;
 }
 
 int check_cells(bh_handle* h, const bh_cells* c) {
@@ -667,6 +726,7 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
   }
   if (sd.on && !persistent)
     return fail(h, BH_ERR_STATE, "only the persistent kernel makes new photons itself");
+  h->last_lattice = lat ? 1 : 0;
   if (h->opt.algorithm == BH_ALGO_EVENT) {
     if (P) return fail(h, BH_ERR_STATE, "the event-split algorithm is not captured as a graph");
     return launch_event(h, W, n, co, s, lds);
@@ -729,10 +789,15 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
       HIPCHK(h, hipGetLastError());
       return launch_replay(h, cells, W, rl, t, shm, co, P, s, sd);
     }
-    if (lat && gen) go(&k_transport_persistent<T, double, true, true>);
-    else if (lat) go(&k_transport_persistent<T, double, true, false>);
-    else if (gen) go(&k_transport_persistent<T, double, false, true>);
-    else go(&k_transport_persistent<T, double, false, false>);
+    if (lat && gen) go(&k_transport_persistent<T, double, 1, true>);
+    else if (lat) go(&k_transport_persistent<T, double, 1, false>);
+    else if (gen) go(&k_transport_persistent<T, double, 0, true>);
+    else if (cells.lat) {  // a batch of problems that are each a lattice
+      h->last_lattice = 2;
+      go(&k_transport_persistent<T, double, 2, false>);
+    } else {
+      go(&k_transport_persistent<T, double, 0, false>);
+    }
     HIPCHK(h, hipGetLastError());
     return BH_OK;
   });
@@ -1067,6 +1132,7 @@ void bh_destroy(bh_handle* h) {
   release(h->comb_cellE);
   release(h->comb_ctr);
   release(h->cflag);
+  release(h->lat);
   release(h->cmark);
   release(h->mwords);
   release(h->mcounts);
@@ -1200,6 +1266,8 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
       }
   // tally_mode 3 forces two tallies per flush
   h->gray_sigma_a = h->uniform_sigma_a && h->opt.tally_mode != 3;
+  // a batch of problems kept across a new mesh of the same size
+  TRY(build_inst_lattice(h, c));
   return BH_OK;
 }
 
@@ -1336,6 +1404,7 @@ int bh_set_instances(bh_handle* h, uint32_t n_instances, const uint32_t* first_c
   HIPCHK(h, hipSetDevice(h->device));
   if (n_instances == 0) {  // back to one problem
     h->n_inst = 0;
+    h->inst_lattice = false;
     h->n_cen = 0;
     h->n_new = 0;
     return BH_OK;
@@ -1382,6 +1451,36 @@ int bh_set_instances(bh_handle* h, uint32_t n_instances, const uint32_t* first_c
   h->n_inst = n_instances;
   h->n_cen = 0;  // a resident census belongs to the previous arrangement
   h->n_new = 0;
+  if (!h->lattice && h->opt.general_mesh == 0) {
+    // the mesh as the device holds it, for the per-instance lattice test
+    const uint32_t n = h->n_cells, G = h->G;
+    std::vector<CellGeo> geo(n);
+    std::vector<double2> sig((size_t)n * G);
+    std::vector<uint32_t> next((size_t)n * 6);
+    HIPCHK(h, hipMemcpyAsync(geo.data(), h->geo.p, n * sizeof(CellGeo), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(sig.data(), h->sigma.p, sig.size() * sizeof(double2),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(next.data(), h->next.p, next.size() * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> nodes((size_t)n * 6), sa((size_t)n * G), ss((size_t)n * G), f(n);
+    std::vector<int32_t> bc((size_t)n * 6);
+    for (uint32_t i = 0; i < n; ++i) {
+      for (int q = 0; q < 6; ++q) {
+        nodes[(size_t)i * 6 + q] = geo[i].nodes[q];
+        bc[(size_t)i * 6 + q] = (int32_t)((geo[i].bc_bits >> (3 * q)) & 7u);
+      }
+      f[i] = geo[i].fleck;
+    }
+    for (size_t k = 0; k < sig.size(); ++k) {
+      sa[k] = sig[k].x;
+      ss[k] = sig[k].y;
+    }
+    const bh_cells view{n, G, nodes.data(), bc.data(), next.data(), sa.data(), ss.data(),
+                        f.data()};
+    TRY(build_inst_lattice(h, &view));
+  }
   return BH_OK;
 }
 
@@ -1486,7 +1585,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
-    stats->lattice = h->lattice ? 1 : 0;
+    stats->lattice = h->last_lattice;
     stats->tally = h->last_tally;
     stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
@@ -1526,6 +1625,7 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
           u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
           u(h->h_tally),     (uintptr_t)h->n_cells, (uintptr_t)h->G,
           u(h->rec.p),       u(h->axes.p),    (uintptr_t)h->lattice,
+          u(h->lat.p),       (uintptr_t)h->inst_lattice,
           u(h->cmark.p),     u(h->mwords.p),  u(h->mcounts.p),      u(h->erank.p),
           u(h->d_cacc),      u(h->sum_part.p),
           u(h->hrec.p),      u(h->hole_len.p), u(h->hole_off.p),   u(h->tail_live.p),
@@ -1711,7 +1811,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
-    stats->lattice = h->lattice ? 1 : 0;
+    stats->lattice = h->last_lattice;
     stats->tally = h->last_tally;
     stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph
@@ -1930,7 +2030,7 @@ int bh_transport_photons(bh_handle* h, bh_photon* photons, uint64_t n, double* a
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
-    stats->lattice = h->lattice ? 1 : 0;
+    stats->lattice = h->last_lattice;
     stats->tally = h->last_tally;
     stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
@@ -2058,7 +2158,7 @@ int bh_transport_photons_noalloc(bh_handle* h, bh_photon* photons, uint64_t n, d
     stats->n_segments = ctrs.segments;
     stats->n_flushes = ctrs.flushes;
     stats->n_replayed = ctrs.replays;
-    stats->lattice = h->lattice ? 1 : 0;
+    stats->lattice = h->last_lattice;
     stats->tally = h->last_tally;
     stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
@@ -2338,7 +2438,8 @@ int resident_physics(bh_handle* h, bool first, uint64_t n_user, double dt, const
   const uint32_t n = h->n_cells;
   const MatDev md = mat_dev(h);
   k_phys_cells<<<grid_for(n, 256), 256, 0, h->stream>>>(
-      h->geo.p, h->sigma.p, h->lattice ? h->rec.p : nullptr, h->G, n, md, dt, h->mat_rf,
+      h->geo.p, h->sigma.p, (h->lattice || h->inst_lattice) ? h->rec.p : nullptr, h->G, n, md,
+      dt, h->mat_rf,
       first ? 1 : 0, h->em.p, h->src.p, h->cen.p, h->mat_e.p, P);
   k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_phys),
                                       sizeof(PhysTotals) / 8);
@@ -2726,7 +2827,7 @@ int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uin
   st.n_segments = ctrs.segments;
   st.n_flushes = ctrs.flushes;
   st.n_replayed = ctrs.replays;
-  st.lattice = h->lattice ? 1 : 0;
+  st.lattice = h->last_lattice;
   st.tally = h->last_tally;
   st.n_allocs = take_allocs(h);
   st.transport_ms = elapsed_ms(h->ev[0], h->ev[1]);  // the whole graph

@@ -91,6 +91,17 @@ struct DevCells {
   const CellRec* rec;
   const double* axis[3];
   uint32_t dims[3];
+  // many problems, each a lattice (else null): per-instance shape and axis
+  // offsets; CellRec::pad holds the cell's instance, axis[0] the axes of
+  // every instance back to back
+  const struct LatInst* lat;
+};
+
+// One problem of a batch whose problems are lattices: its first cell, its
+// shape, and where its x, y and z nodes start in the concatenated axes.
+struct LatInst {
+  uint32_t first, nx, ny, nz;
+  uint32_t ox, oy, oz, pad;
 };
 
 // The RNG counter-high word of a photon in cell c.

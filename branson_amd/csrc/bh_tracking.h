@@ -125,16 +125,18 @@ constexpr float kU = 0x1p-24f;
 constexpr float kDir = 0x1p-22f;
 constexpr float kE = 0x1p-21f;
 
-// The data one segment reads about its cell.  kLat = false: the 64-byte
-// CellGeo record and the per-group opacities; kLat = true (lattice meshes):
+// The data one segment reads about its cell.  kLat = 0: the 64-byte
+// CellGeo record and the per-group opacities; kLat = 1 (lattice meshes):
 // the 32-byte CellRec and the axis node arrays, indexed by the photon's
-// (i, j, k).  node(k) is the face coordinate k = 2 * axis + (high side).
-// Both views return the same numbers for the same cell.
-template <bool kLat>
+// (i, j, k); kLat = 2 (a batch of problems that are each a lattice): the
+// CellRec and the axes of the cell's problem.  node(k) is the face
+// coordinate k = 2 * axis + (high side).  All views return the same numbers
+// for the same cell.
+template <int kLat>
 struct CellView;
 
 template <>
-struct CellView<false> {
+struct CellView<0> {
   const CellGeo* geo;
   const double2* sig;
   double f;
@@ -154,7 +156,7 @@ __device__ __forceinline__ T pick3(T a, T b, T c, uint32_t i) {
 }
 
 template <>
-struct CellView<true> {
+struct CellView<1> {
   const double* ax0;
   const double* ax1;
   const double* ax2;
@@ -177,20 +179,57 @@ struct CellView<true> {
   __device__ __forceinline__ uint32_t bc_bits() const { return bits; }
 };
 
-// (i, j, k) of cell c on a lattice numbered x fastest.
+template <>
+struct CellView<2> {
+  const double* ax;      // every problem's axes
+  uint32_t i0, i1, i2;   // the cell's low nodes in ax
+  uint32_t nx, nxy;      // index strides of the cell's problem
+  double sa, ss, f;
+  uint32_t bits;
+  __device__ __forceinline__ CellView(const DevCells& c, uint32_t cell, const uint32_t* ijk)
+      : ax(c.axis[0]) {
+    const CellRec r = c.rec[cell];
+    sa = r.sigma_a;
+    ss = r.sigma_s;
+    f = r.fleck;
+    bits = r.bc_bits;
+    const LatInst li = c.lat[r.pad];
+    i0 = li.ox + ijk[0];
+    i1 = li.oy + ijk[1];
+    i2 = li.oz + ijk[2];
+    nx = li.nx;
+    nxy = li.nx * li.ny;
+  }
+  __device__ __forceinline__ double2 sigma(uint32_t) const { return make_double2(sa, ss); }
+  __device__ __forceinline__ double node(uint32_t k) const {
+    return ax[pick3(i0, i1, i2, k >> 1) + (k & 1u)];
+  }
+  __device__ __forceinline__ uint32_t bc_bits() const { return bits; }
+};
+
+// (i, j, k) of cell c on a lattice numbered x fastest (kLat = 2: within the
+// cell's problem).
+template <int kLat = 1>
 __device__ __forceinline__ void lattice_ijk(const DevCells& cells, uint32_t c, uint32_t ijk[3]) {
-  const uint32_t nxy = cells.dims[0] * cells.dims[1];
+  uint32_t nx = cells.dims[0], ny = cells.dims[1];
+  if constexpr (kLat == 2) {
+    const LatInst li = cells.lat[cells.cell_inst[c]];
+    c -= li.first;
+    nx = li.nx;
+    ny = li.ny;
+  }
+  const uint32_t nxy = nx * ny;
   ijk[2] = c / nxy;
   const uint32_t r = c - ijk[2] * nxy;
-  ijk[1] = r / cells.dims[0];
-  ijk[0] = r - ijk[1] * cells.dims[0];
+  ijk[1] = r / nx;
+  ijk[0] = r - ijk[1] * nx;
 }
 
-template <typename R, bool kLat = false>
+template <typename R, int kLat = 0>
 __device__ __forceinline__ void load_live(const DevCells& cells, const PhotonSoA& W, uint64_t i,
                                           uint64_t ctr_hi, Live<R>& L) {
   L.cell = W.cell[i];
-  if constexpr (kLat) lattice_ijk(cells, L.cell, L.ijk);
+  if constexpr (kLat) lattice_ijk<kLat>(cells, L.cell, L.ijk);
   if constexpr (sizeof(R) == 4) {
     const CellView<kLat> cv(cells, L.cell, L.ijk);
     L.frame = L.cell;
@@ -227,7 +266,7 @@ __device__ __forceinline__ void load_live(const DevCells& cells, const PhotonSoA
   L.pending = false;
 }
 
-template <typename R, bool kLat = false>
+template <typename R, int kLat = 0>
 __device__ __forceinline__ void store_live(const DevCells& cells, const PhotonSoA& W, uint64_t i,
                                            const Live<R>& L) {
   if constexpr (sizeof(R) == 4) {
@@ -275,11 +314,16 @@ __device__ __forceinline__ void flush(Flush& fo, const Live<R>& L) {
 // not be a cell of this mesh, so there the frame stays with the old cell.
 // The neighbour across face sc (ELEMENT or PROCESSOR); on a lattice the
 // neighbour is the next index along the face's axis.
-template <bool kLat, typename R>
-__device__ __forceinline__ uint32_t neighbour(const DevCells& cells, const Live<R>& L) {
+template <int kLat, typename R>
+__device__ __forceinline__ uint32_t neighbour(const DevCells& cells, const CellView<kLat>& cv,
+                                              const Live<R>& L) {
   if constexpr (kLat) {
     const uint32_t a = L.sc >> 1;
-    const uint32_t stride = (a == 0) ? 1u : (a == 1) ? cells.dims[0] : cells.dims[0] * cells.dims[1];
+    uint32_t stride;
+    if constexpr (kLat == 2)
+      stride = (a == 0) ? 1u : (a == 1) ? cv.nx : cv.nxy;
+    else
+      stride = (a == 0) ? 1u : (a == 1) ? cells.dims[0] : cells.dims[0] * cells.dims[1];
     return (L.sc & 1u) ? L.cell + stride : L.cell - stride;
   } else {
     return cells.next[(size_t)L.cell * 6 + L.sc];
@@ -300,7 +344,7 @@ __device__ __forceinline__ uint32_t lattice_step(Live<R>& L, uint32_t* from) {
   return i1;
 }
 
-template <bool kLat>
+template <int kLat>
 __device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView<kLat>&,
                                            uint32_t nc, Live<double>& L) {
   if constexpr (kLat) {
@@ -309,7 +353,7 @@ __device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView
   }
   L.cell = nc;
 }
-template <bool kLat>
+template <int kLat>
 __device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView<kLat>& from,
                                            uint32_t nc, Live<float>& L) {
   float sx, sy, sz;
@@ -341,9 +385,9 @@ __device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView
 // A reflection mirrors both the exact and the FP32 continuation about the
 // face; the two mirror planes differ by the FP32 rounding of the face
 // offset, which is all a reflection adds to the error.
-template <bool kLat>
+template <int kLat>
 __device__ __forceinline__ void reflect_error(const CellView<kLat>&, uint32_t, Live<double>&) {}
-template <bool kLat>
+template <int kLat>
 __device__ __forceinline__ void reflect_error(const CellView<kLat>& cv, uint32_t axis,
                                               Live<float>& L) {
   const float w = (float)(cv.node(2 * axis + 1) - cv.node(2 * axis));
@@ -351,20 +395,20 @@ __device__ __forceinline__ void reflect_error(const CellView<kLat>& cv, uint32_t
   L.eT += 2.0f * kU * w;
 }
 
-template <bool kLat, typename R>
+template <int kLat, typename R>
 __device__ __forceinline__ bool boundary_event(const DevCells& cells, const CellView<kLat>& cv,
                                                Live<R>& L, Flush& fo) {
   const int bc = face_bc(cv.bc_bits(), L.sc);
   if (bc == kElement) {
     flush(fo, L);
-    enter_cell(cells, cv, neighbour<kLat>(cells, L), L);
+    enter_cell(cells, cv, neighbour<kLat>(cells, cv, L), L);
     L.desc = kBound;
     L.t_abs = 0.0;
     L.t_track = 0.0;
     return false;
   }
   if (bc == kProcessor) {  // (never on a lattice mesh)
-    const uint32_t nc = neighbour<kLat>(cells, L);
+    const uint32_t nc = neighbour<kLat>(cells, cv, L);
     L.desc = kPass;
     flush(fo, L);
     L.cell = nc;
@@ -496,11 +540,11 @@ __device__ __forceinline__ void wave_gen_draws(bool gen, int nd, uint64_t stream
 
 // A new photon's initial state in the registers of a tracking kernel, as
 // load_live would read it from the store.
-template <typename R, bool kLat>
+template <typename R, int kLat>
 __device__ __forceinline__ void live_from_source(const DevCells& cells, const SourcePhoton& o,
                                                  uint64_t hi, Live<R>& L) {
   L.cell = o.cell;
-  if constexpr (kLat) lattice_ijk(cells, L.cell, L.ijk);
+  if constexpr (kLat) lattice_ijk<kLat>(cells, L.cell, L.ijk);
   if constexpr (sizeof(R) == 4) {
     const CellView<kLat> cv(cells, L.cell, L.ijk);
     L.frame = L.cell;
@@ -539,7 +583,7 @@ __device__ __forceinline__ void live_from_source(const DevCells& cells, const So
 
 // New photon p made by one lane alone (the FP64 replay): its draws one by
 // one, then the source's sampling.
-template <typename R, bool kLat>
+template <typename R, int kLat>
 __device__ __forceinline__ void gen_photon_lane(const DevCells& cells, const SourceDesc& sd,
                                                 uint64_t n_new, double dt, uint64_t p,
                                                 Live<R>& L) {
@@ -573,7 +617,7 @@ struct SegInfo {
 // 95-100); then, with `distance`, the distance to scatter (:62-63), which is
 // returned (1e100 when there is none).  sg / total_sigma_s follow a group
 // change.
-template <bool kInfo, bool kLat>
+template <bool kInfo, int kLat>
 __device__ __forceinline__ double lane_draws(const DevCells& cells, const CellView<kLat>& cv,
                                              double f, Live<double>& L, double2& sg,
                                              double& total_sigma_s, SegInfo* si, bool distance) {
@@ -634,7 +678,7 @@ __device__ __forceinline__ double lane_draws(const DevCells& cells, const CellVi
 // pre: the cell's view loaded before the wave's draws, so that the load's
 // latency hides behind them (the draws' LDS fences keep the compiler from
 // moving it there itself); null: loaded here.
-template <bool kTrack, bool kLat = false, bool kInfo = false>
+template <bool kTrack, int kLat = 0, bool kInfo = false>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, Flush& fo,
                                         SegInfo* si = nullptr, WaveDraws* dr = nullptr,
                                         const CellView<kLat>* pre = nullptr) {
@@ -793,7 +837,7 @@ __device__ __forceinline__ bool segment(const DevCells& cells, Live<double>& L, 
 // could be off by more than fp32_tol (relative) is treated the same way, so
 // every FP32 deposit -- and with it every cell's tally -- is within fp32_tol
 // of FP64.
-template <bool kTrack, bool kLat = false>
+template <bool kTrack, int kLat = 0>
 __device__ __forceinline__ bool segment(const DevCells& cells, Live<float>& L, Flush& fo,
                                         SegInfo* = nullptr, WaveDraws* dr = nullptr,
                                         const CellView<kLat>* pre = nullptr) {
@@ -1090,7 +1134,7 @@ __device__ __forceinline__ void replay_append(bool want, uint64_t idx, uint32_t 
 // End of a history inside a tracking kernel: store the photon, or -- FP32
 // form, ambiguous decision -- leave its initial state in place for the
 // replay.  Returns the segments this kernel applied.
-template <typename R, bool kLat>
+template <typename R, int kLat>
 __device__ __forceinline__ uint32_t end_history(const DevCells& cells, const PhotonSoA& W,
                                                 uint64_t i, const Live<R>& L, bool& replay,
                                                 bool in_place) {
@@ -1207,7 +1251,7 @@ struct CensusLocal {
 // the source type comes from the lane (kernels that make new photons);
 // otherwise from the photon's slot of W, so that the lane does not carry it
 // through the history.
-template <typename R, bool kLat, bool kDense = false, bool kLiveType = false>
+template <typename R, int kLat, bool kDense = false, bool kLiveType = false>
 __device__ __forceinline__ void census_append(bool ended, const DevCells& cells, uint64_t i,
                                               const Live<R>& L, const CensusRun& cr,
                                               CensusLocal& loc) {
@@ -1317,7 +1361,7 @@ __device__ __forceinline__ void census_flush(const CensusLocal& loc) {
   }
 }
 
-template <class Tally, typename R, bool kLat>
+template <class Tally, typename R, int kLat>
 BH_TRACK_KERNEL void k_transport_history(const PhotonSoA W, const CensusOut co,
                                          DevCells cells, uint64_t n, uint64_t ctr_hi,
                                          Tally tally, DevCounters* ctrs, ReplayList rl) {
@@ -1369,7 +1413,7 @@ BH_TRACK_KERNEL void k_transport_history(const PhotonSoA W, const CensusOut co,
 // The generating instances (kGen) hold the source sampling's temporaries on
 // top of the tracking state: at 3 waves per SIMD they spilled 216-248 B per
 // lane; at 2 they keep everything in 205-210 VGPRs and measured 12% faster.
-template <class Tally, typename R, bool kLat, bool kGen>
+template <class Tally, typename R, int kLat, bool kGen>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(kGen ? 2 : BH_TRACK_WAVES_PER_EU)))
 void k_transport_persistent(const PhotonSoA W, const CensusOut co,
@@ -1546,7 +1590,7 @@ __global__ void k_ev_clear(EventState ev, int nxt) {
   ev.count[2] = 0;
 }
 
-template <class Tally, bool kLat>
+template <class Tally, int kLat>
 BH_TRACK_KERNEL void k_ev_flight(const PhotonSoA W, const CensusOut co, DevCells cells,
                                  EventState ev, int cur, uint64_t ctr_hi, Tally tally,
                                  DevCounters* ctrs) {
@@ -1606,7 +1650,7 @@ BH_TRACK_KERNEL void k_ev_flight(const PhotonSoA W, const CensusOut co, DevCells
   if (co.on) census_flush(loc);
 }
 
-template <bool kLat>
+template <int kLat>
 __global__ __launch_bounds__(256) void k_ev_scatter(PhotonSoA W, DevCells cells, EventState ev,
                                                     int cur, uint64_t ctr_hi) {
   const uint64_t n = ev.count[2];
@@ -1620,7 +1664,7 @@ __global__ __launch_bounds__(256) void k_ev_scatter(PhotonSoA W, DevCells cells,
       Live<double> L;
       L.cell = W.cell[i];
       L.group = W.group[i];
-      if constexpr (kLat) lattice_ijk(cells, L.cell, L.ijk);
+      if constexpr (kLat) lattice_ijk<kLat>(cells, L.cell, L.ijk);
       L.rng.ctr = W.ctr[i];
       L.rng.stream = W.stream[i];
       L.rng.ctr_hi = photon_ctr_hi(cells, L.cell, ctr_hi);

@@ -134,7 +134,10 @@ typedef struct bh_step_stats {
                              the FP32 error bound could not separate,
                              finished in FP64 from that segment on        */
   int32_t lattice;        /* 1: tracked on the lattice records (see
-                             bh_options.general_mesh), 0: general records */
+                             bh_options.general_mesh), 2: on per-instance
+                             lattice records (bh_set_instances, every
+                             instance a lattice; FP64 persistent waves),
+                             0: general records                          */
   int32_t tally;          /* tally form of the last transport: 1 one global
                              atomic per flush, 2 block-private LDS, 3 two
                              global atomics per flush, 4 wave hash table   */

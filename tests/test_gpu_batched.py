@@ -110,6 +110,33 @@ def test_batched_transport_fp64_matches_oracle(algo):
     assert np.allclose(t_g, t_o, rtol=1e-12, atol=1e-300)
 
 
+def test_batched_instance_lattice_records_equal_general_records():
+    """Every problem of the batch is a lattice, so the persistent FP64 kernel
+    tracks on per-instance 32-byte records and axes (stats lattice 2).  The
+    same arithmetic runs on the same numbers as on the general records
+    (general_mesh), so every photon agrees bit for bit and the tallies to
+    summation order."""
+    b = _small_batch()
+    em, total_E = _host_energies(b)
+    n, dt = b.n_cells, b.parts[0].dt
+    out = {}
+    for general in (False, True):
+        with _engine_with_batch(b, algorithm=capi.ALGO_PERSISTENT, general_mesh=general) as e:
+            e.source_instances(em, np.zeros(n), None, np.full(n, -1, dtype=np.int8),
+                               b.n_photons, total_E, 1, 0, dt)
+            ph = e.photons()
+            a, t, st = e.transport_photons(ph)
+            out[general] = (ph, a, t, st)
+    (pl, al, tl, sl), (pg, ag, tg, sg) = out[False], out[True]
+    assert sl["lattice"] == 2 and sg["lattice"] == 0
+    for f in ("descriptor", "cell", "group", "rng", "pos", "angle", "E", "life_dx"):
+        assert np.array_equal(pl[f], pg[f]), f
+    for k in ("n_segments", "n_census", "n_exit", "n_killed", "n_flushes"):
+        assert sl[k] == sg[k], k
+    assert np.allclose(al, ag, rtol=1e-12, atol=0)
+    assert np.allclose(tl, tg, rtol=1e-12, atol=0)
+
+
 def test_batched_fp32_tracking_events_match_fp64():
     b = _small_batch()
     em, total_E = _host_energies(b)
