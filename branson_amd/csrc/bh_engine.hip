@@ -955,8 +955,8 @@ int materialize_source(bh_handle* h, uint64_t n_new) {
   TRY(grow_store(h, h->W, h->n_cen + n_new, h->n_cen));
   const SourceDesc& d = h->sd;
   k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
-      dev_cells(h), d.em, d.src, d.face, d.off, n_new, d.one, d.inst, d.dt, 0, h->W.soa,
-      h->n_cen, nullptr);
+      dev_cells(h), d.em, d.src, d.face, d.off, d.hint, n_new, d.one, d.inst, d.dt, 0,
+      h->W.soa, h->n_cen, nullptr);
   HIPCHK(h, hipGetLastError());
   h->src_deferred = false;
   return BH_OK;
@@ -1345,8 +1345,8 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
     TRY(grow_store(h, h->W, n_c, 0));
     if (n_c)
       k_source_fill<<<grid_for(n_c, 256), 256, 0, h->stream>>>(
-          cells, h->cen.p, nullptr, h->face.p, h->soff.p, n_c, census, icen, dt, 1, h->W.soa, 0,
-          nullptr);
+          cells, h->cen.p, nullptr, h->face.p, h->soff.p, nullptr, n_c, census, icen, dt, 1,
+          h->W.soa, 0, nullptr);
     HIPCHK(h, hipGetLastError());
     h->n_cen = n_c;
     h->has_census_E = false;
@@ -1649,8 +1649,8 @@ static int record_step(bh_handle* h, uint64_t cap, bool gen) {
                    h->stream>>>(h->soff.p, nc, cap, &P->n_new, h->src_hint.p);
   if (!gen)
     k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
-        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, 0, unused, &P->fresh, 0.0, 0,
-        h->W.soa, 0, P);
+        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused,
+        &P->fresh, 0.0, 0, h->W.soa, 0, P);
   HIPCHK(h, hipGetLastError());
   const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, &P->fresh,
                       unused,  0.0,      cap,       gen ? 1 : 0, 0};
@@ -2681,8 +2681,8 @@ int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0, bool gen) {
                    h->stream>>>(h->soff.p, nc, cap, &P->n_new, h->src_hint.p);
   if (!gen)
     k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
-        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, 0, unused, inew, 0.0, 0,
-        h->W.soa, 0, P);
+        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused, inew,
+        0.0, 0, h->W.soa, 0, P);
   HIPCHK(h, hipGetLastError());
   const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, inew,
                       unused,  0.0,      cap,       gen ? 1 : 0, 0};

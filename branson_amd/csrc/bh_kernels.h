@@ -441,32 +441,47 @@ __device__ __forceinline__ uint32_t hinted_cell(const SourceDesc& sd, uint32_t n
   return cell_of_photon(sd.off, p, lo, hi);
 }
 
-// A block's 256 consecutive photons fall in a few consecutive cells: two
-// waves find the cells of its first and last photon over the whole offset
-// array, and each photon then searches only that range (the same cell the
-// full search gives, in one or two steps instead of ~16 dependent loads).
+// A block's 256 consecutive photons fall in a few consecutive cells.  With
+// `hint` (k_source_hints over the same offsets and count, kHintStride = the
+// block size), the block's first photon p0 is a multiple of kHintStride, so
+// hint[p0 / kHintStride] is its cell and hint[p0 / kHintStride + 1] bounds
+// the cell of its last: one scalar load pair per block, then each photon
+// searches that range (one or two steps where a cell holds many photons).
+// Without hints (the initial census), two waves find the cells of the
+// block's first and last photon over the whole offset array (~18 dependent
+// loads at 64^3 cells, whose latency the rest of the block waits out).
 __global__ __launch_bounds__(256) void k_source_fill(
     DevCells cells, const double* __restrict__ emission_E, const double* __restrict__ source_E,
-    const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off, uint64_t n_new,
-    SourceInst one, const SourceInst* __restrict__ inst, double dt, int mode, PhotonSoA W,
-    uint64_t out_base, const StepParams* __restrict__ P) {
+    const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ hint, uint64_t n_new, SourceInst one,
+    const SourceInst* __restrict__ inst, double dt, int mode, PhotonSoA W, uint64_t out_base,
+    const StepParams* __restrict__ P) {
   __shared__ uint32_t s_first, s_last;
   if (P) {
     out_base = P->n_cen;
     n_new = P->n_new;
     dt = P->dt;
   }
+  const uint64_t n_hint = (n_new + kHintStride - 1) / kHintStride;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t p0 = (uint64_t)blockIdx.x * blockDim.x; p0 < n_new; p0 += stride) {
-    const uint64_t p_last = (p0 + blockDim.x < n_new ? p0 + blockDim.x : n_new) - 1;
-    if (threadIdx.x == 0) s_first = cell_of_photon(off, p0, 0u, cells.n_cells);
-    if (threadIdx.x == kWave) s_last = cell_of_photon(off, p_last, 0u, cells.n_cells);
-    __syncthreads();
-    const uint32_t c_first = s_first, c_last = s_last;
-    __syncthreads();
+    uint32_t c_first, c_end;
+    if (hint) {
+      const uint64_t j = p0 / kHintStride;
+      c_first = hint[j];
+      c_end = (j + 1 < n_hint) ? hint[j + 1] + 1u : cells.n_cells;
+    } else {
+      const uint64_t p_last = (p0 + blockDim.x < n_new ? p0 + blockDim.x : n_new) - 1;
+      if (threadIdx.x == 0) s_first = cell_of_photon(off, p0, 0u, cells.n_cells);
+      if (threadIdx.x == kWave) s_last = cell_of_photon(off, p_last, 0u, cells.n_cells);
+      __syncthreads();
+      c_first = s_first;
+      c_end = s_last + 1u;
+      __syncthreads();
+    }
     const uint64_t p = p0 + threadIdx.x;
     if (p < n_new) {
-      const uint32_t c = cell_of_photon(off, p, c_first, c_last + 1u);
+      const uint32_t c = cell_of_photon(off, p, c_first, c_end);
       source_one(cells, emission_E, source_E, face_of, off, p, c, one, inst, dt, mode, W,
                  out_base);
     }
