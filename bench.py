@@ -521,7 +521,7 @@ def _bench_cube(a, capi):
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK_GBPS,
                   "traffic": _traffic_from_profile(CUBE_TRAFFIC, a.steps, a.warmup),
-                  "kernel": "k_transport_persistent<GlobalTally<false>, double, true> "
+                  "kernel": "k_transport_persistent<GlobalTally<false>, double, 1> "
                             "(plain fp64, cube)",
                   "kernel_ms_mean": kms, "algorithmic_bytes_per_launch": bytes_launch,
                   "launches": len(plain["k_ms"]),
