@@ -803,13 +803,43 @@ int launch_transport(bh_handle* h, const PhotonSoA& W, uint64_t n, const StepPar
   });
 }
 
-int zero_tallies_and_counters(bh_handle* h) {
-  k_fill_f64<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(h->abs_E.p, h->n_cells, 0.0);
-  k_fill_f64<<<grid_for(h->n_cells, 256), 256, 0, h->stream>>>(h->track_E.p, h->n_cells, 0.0);
-  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_ctrs),
-                                      sizeof(DevCounters) / 8);
-  HIPCHK(h, hipGetLastError());
+// Arrays to clear before a launch, cleared together by one k_zero_segs.
+struct ZeroList {
+  ZeroSegs z{};
+  uint64_t max_words = 0;
+};
+
+int zero_launch(bh_handle* h, ZeroList& zl) {
+  if (zl.z.count) {
+    const unsigned gx =
+        (unsigned)std::min<uint64_t>(std::max<uint64_t>(div_up(zl.max_words, 256), 1), 2048);
+    k_zero_segs<<<dim3(gx, zl.z.count), 256, 0, h->stream>>>(zl.z);
+    HIPCHK(h, hipGetLastError());
+  }
+  zl = ZeroList{};
   return BH_OK;
+}
+
+int zero_add(bh_handle* h, ZeroList& zl, void* p, size_t bytes) {
+  if (zl.z.count == kZeroSegs) TRY(zero_launch(h, zl));
+  const uint64_t w = bytes / 8;  // every array cleared here is 8-byte words
+  zl.z.p[zl.z.count] = static_cast<unsigned long long*>(p);
+  zl.z.n[zl.z.count] = w;
+  ++zl.z.count;
+  zl.max_words = std::max(zl.max_words, w);
+  return BH_OK;
+}
+
+// The tallies and counters of a launch cleared: now, or (pending) with the
+// census buffers by census_prepare.
+int zero_tallies_and_counters(bh_handle* h, ZeroList* pending = nullptr) {
+  ZeroList own;
+  ZeroList& zl = pending ? *pending : own;
+  TRY(zero_add(h, zl, h->abs_E.p, (size_t)h->n_cells * sizeof(double)));
+  TRY(zero_add(h, zl, h->track_E.p, (size_t)h->n_cells * sizeof(double)));
+  static_assert(sizeof(DevCounters) % 8 == 0, "counters are 64-bit words");
+  TRY(zero_add(h, zl, h->d_ctrs, sizeof(DevCounters)));
+  return pending ? BH_OK : zero_launch(h, zl);
 }
 
 // Capacities of a census-writing launch over at most n photons.  Every
@@ -865,7 +895,8 @@ int ensure_marks(bh_handle* h, uint64_t pc) {
 // the waves leave, the flags by transport index are cleared, the totals
 // zeroed.  Fill kernels, no memset, so the same code records into a graph
 // (P: sizes read on the device; cap bounds n).
-int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co) {
+int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co,
+                   ZeroList* pending = nullptr) {
   // census keys are 32-bit transport indices and ranks, 0xFFFFFFFF marks a hole
   if (cap >= (uint64_t)kHole)
     return fail(h, BH_ERR_CAPACITY, "%llu photons in one launch: at most %u are supported",
@@ -879,11 +910,12 @@ int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co) {
   TRY(ensure(h, h->sum_part, (size_t)8 * kSumMaxBlocks));
   TRY(ensure(h, h->hrec, z.rec_cap));
   const uint64_t mw = mark_words(cap);  // the marks of this launch and a zero word
-  k_fill_u64<<<grid_for(mw * 8, 256), 256, 0, h->stream>>>(
-      reinterpret_cast<unsigned long long*>(h->cmark.p), mw * 8);
-  k_fill_u64<<<1, 64, 0, h->stream>>>(reinterpret_cast<unsigned long long*>(h->d_cacc),
-                                      sizeof(CensusAcc) / 8);
-  HIPCHK(h, hipGetLastError());
+  ZeroList own;
+  ZeroList& zl = pending ? *pending : own;
+  TRY(zero_add(h, zl, h->cmark.p, (size_t)mw * 64));
+  static_assert(sizeof(CensusAcc) % 8 == 0, "census totals are 64-bit words");
+  TRY(zero_add(h, zl, h->d_cacc, sizeof(CensusAcc)));
+  TRY(zero_launch(h, zl));
   co = CensusOut{h->C.soa, reinterpret_cast<uint8_t*>(h->cmark.p), h->d_cacc, h->hrec.p,
                  kC * next_dt, h->n_cen, h->n_new, h->C.cap, z.rec_cap, 1};
   return BH_OK;
@@ -1522,12 +1554,13 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
   if (!h->has_mesh) return fail(h, BH_ERR_STATE, "bh_transport before bh_set_mesh");
   HIPCHK(h, hipSetDevice(h->device));
   const uint64_t n = h->n_cen + h->n_new;
-  TRY(zero_tallies_and_counters(h));
+  ZeroList zl;
+  TRY(zero_tallies_and_counters(h, &zl));
   // the kernels write the census to C themselves (post_process_photons,
   // src/post_process_functions.h:32-109); the reference's census order is
   // restored from the transport indices afterwards
   CensusOut co;
-  TRY(census_prepare(h, n, next_dt, co));
+  TRY(census_prepare(h, n, next_dt, co, &zl));
   co.inst_exit = (h->inst_exit_on && h->n_inst) ? h->inst_exit.p : nullptr;
   h->replay_timed = false;
   // new photons not yet in W are made inside the persistent kernel
@@ -1654,9 +1687,10 @@ static int record_step(bh_handle* h, uint64_t cap, bool gen) {
   HIPCHK(h, hipGetLastError());
   const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, &P->fresh,
                       unused,  0.0,      cap,       gen ? 1 : 0, 0};
-  TRY(zero_tallies_and_counters(h));
+  ZeroList zl;
+  TRY(zero_tallies_and_counters(h, &zl));
   CensusOut co;  // n_cen, n_new and the census life come from P on the device
-  TRY(census_prepare(h, cap, 0.0, co));
+  TRY(census_prepare(h, cap, 0.0, co, &zl));
   TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi), sd));
   TRY(census_finish(h, cap, &P->n));
   char* hp = static_cast<char*>(h->h_pinned);
@@ -2692,9 +2726,10 @@ int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0, bool gen) {
                                  h->cell_inst.p, h->inst_rep.p, 0);
     HIPCHK(h, hipGetLastError());
   }
-  TRY(zero_tallies_and_counters(h));
+  ZeroList zl;
+  TRY(zero_tallies_and_counters(h, &zl));
   CensusOut co;
-  TRY(census_prepare(h, cap, 0.0, co));
+  TRY(census_prepare(h, cap, 0.0, co, &zl));
   co.inst_exit = K ? h->inst_exit.p : nullptr;
   TRY(launch_transport(h, h->W.soa, cap, P, co, handle_sinks(h, h->ctr_hi), sd));
   TRY(census_finish(h, cap, &P->n));

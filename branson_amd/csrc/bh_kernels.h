@@ -358,6 +358,26 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+// Up to kZeroSegs arrays of 64-bit words cleared by one launch (blockIdx.y
+// picks the array, a grid-stride loop over it in x): the tallies, counters,
+// census marks and totals a step clears before its transport launch, which
+// were five fill launches back to back.
+constexpr uint32_t kZeroSegs = 6;
+struct ZeroSegs {
+  unsigned long long* p[kZeroSegs];
+  uint64_t n[kZeroSegs];
+  uint32_t count;
+};
+
+__global__ __launch_bounds__(256) void k_zero_segs(const ZeroSegs z) {
+  if (blockIdx.y >= z.count) return;
+  unsigned long long* __restrict__ p = z.p[blockIdx.y];
+  const uint64_t n = z.n[blockIdx.y];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = 0ull;
+}
+
 // ---------------------------------------------------------------------------
 // Source (src/source.h).
 // Photons per cell = max(1, int(n_user * E / E_tot)) when E > 0.

@@ -356,6 +356,7 @@ __device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView
 template <int kLat>
 __device__ __forceinline__ void enter_cell(const DevCells& cells, const CellView<kLat>& from,
                                            uint32_t nc, Live<float>& L) {
+  static_assert(kLat != 2, "FP32 tracking runs on the general or the single-lattice records");
   float sx, sy, sz;
   if constexpr (kLat) {  // only the crossing axis moves
     const uint32_t a = L.sc >> 1;
