@@ -235,7 +235,9 @@ struct bh_handle {
   size_t h_tally_cap = 0;
   std::vector<double> em_host, src_host;  // the step's energies, for exact counts
 
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[9] = {};          // 6, 7: the source phase (run_source); 8: its count read
+  bool src_ms_pending = false;    // last_source_ms not yet read from ev[6], ev[7]
+  uint64_t last_n_new = 0;        // new photons of the last run_source (its fill's grid)
   bool replay_timed = false;  // ev[3] marks the start of the last FP64 replay
   bool inst_exit_on = false;  // bh_transport adds exit energy per instance
   double last_source_ms = 0.0;
@@ -988,7 +990,7 @@ int materialize_source(bh_handle* h, uint64_t n_new) {
   const SourceDesc& d = h->sd;
   k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
       dev_cells(h), d.em, d.src, d.face, d.off, d.hint, n_new, d.one, d.inst, d.dt, 0,
-      h->W.soa, h->n_cen, nullptr);
+      h->W.soa, h->n_cen, nullptr, nullptr);
   HIPCHK(h, hipGetLastError());
   h->src_deferred = false;
   return BH_OK;
@@ -1362,7 +1364,7 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
   TRY(ensure(h, h->counts, n));
   TRY(ensure(h, h->soff, n));
   const uint32_t* cinst = h->n_inst ? h->cell_inst.p : nullptr;
-  HIPCHK(h, hipEventRecord(h->ev[4], h->stream));
+  HIPCHK(h, hipEventRecord(h->ev[6], h->stream));
   if (h->has_census_E) {  // step 1: initial census replaces the resident one
     k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->cen.p, nullptr, n, census, icen,
                                                               cinst, 1, h->counts.p);
@@ -1378,7 +1380,7 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
     if (n_c)
       k_source_fill<<<grid_for(n_c, 256), 256, 0, h->stream>>>(
           cells, h->cen.p, nullptr, h->face.p, h->soff.p, nullptr, n_c, census, icen, dt, 1,
-          h->W.soa, 0, nullptr);
+          h->W.soa, 0, nullptr, nullptr);
     HIPCHK(h, hipGetLastError());
     h->n_cen = n_c;
     h->has_census_E = false;
@@ -1386,25 +1388,65 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
   k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, n, fresh, inew,
                                                             cinst, 0, h->counts.p);
   TRY(scan_u64(h, h->counts.p, n, h->soff.p, 1));
-  uint64_t n_new = 0;
-  TRY(read_total(h, 1, &n_new));
-  // the new photons: made inside the persistent transport kernel, or
-  // written to W behind the census here
-  TRY(ensure(h, h->src_hint, div_up(std::max<uint64_t>(n_new, 1), kHintStride) + 1));
+  // The new photons: made inside the persistent transport kernel, or written
+  // to W behind the census here.  Written here, the hints and the fill are
+  // launched before the host reads the count: they read it on the device,
+  // bounded by the room W has, so the GPU works while the host waits for the
+  // count.  A count beyond that room grows W and writes them again.
+  const bool write_now = !source_in_transport(h, 0);
+  const uint64_t room = (write_now && h->W.cap > h->n_cen) ? h->W.cap - h->n_cen : 0;
+  if (room) TRY(ensure(h, h->src_hint, div_up(room, kHintStride) + 1));
+  uint64_t* hp = static_cast<uint64_t*>(h->h_pinned);
+  HIPCHK(h, hipMemcpyAsync(hp, h->d_total + 1, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipEventRecord(h->ev[8], h->stream));  // the count is on the host
+  if (room) {
+    // one 256-photon tile per block measured 8% faster than a grid-stride
+    // grid of 16 blocks per CU: the grid covers the last step's count, or
+    // the photons asked for, and an eighth more (the kernel's grid-stride
+    // loop takes any photons beyond)
+    const uint64_t guess = h->last_n_new ? h->last_n_new : (inew ? room : fresh.n_user);
+    const uint64_t est = guess + guess / 8 + 1;
+    k_source_hints<<<std::min<unsigned>(grid_for(div_up(room, kHintStride), 256), 1024), 256, 0,
+                     h->stream>>>(h->soff.p, n, room, h->d_total + 1, h->src_hint.p);
+    k_source_fill<<<grid_for(std::min(est, room), 256), 256, 0, h->stream>>>(
+        cells, h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, room, fresh, inew, dt, 0,
+        h->W.soa, h->n_cen, nullptr, h->d_total + 1);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipEventSynchronize(h->ev[8]));
+  const uint64_t n_new = *hp;
+  const bool again = n_new > room;  // (or the photons are made in the transport kernel)
+  if (again && room) HIPCHK(h, hipStreamSynchronize(h->stream));  // the short fill is done
+  if (again) TRY(ensure(h, h->src_hint, div_up(n_new, kHintStride) + 1));
   h->sd = SourceDesc{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, inew, fresh, dt,
                      n_new, 1, 0};
-  h->src_deferred = n_new > 0;
-  if (n_new)
+  h->src_deferred = n_new > 0 && (again || !write_now);
+  if (again) {
     k_source_hints<<<std::min<unsigned>(grid_for(div_up(n_new, kHintStride), 256), 1024), 256, 0,
                      h->stream>>>(h->soff.p, n, n_new, nullptr, h->src_hint.p);
-  HIPCHK(h, hipGetLastError());
-  if (!source_in_transport(h, n_new)) TRY(materialize_source(h, n_new));
-  HIPCHK(h, hipEventRecord(h->ev[5], h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->last_source_ms = elapsed_ms(h->ev[4], h->ev[5]);
+    HIPCHK(h, hipGetLastError());
+    if (write_now) TRY(materialize_source(h, n_new));
+  }
+  // the phase's time is read once its end has passed (source_ms), so the
+  // host goes on to queue the transport while the fill runs
+  HIPCHK(h, hipEventRecord(h->ev[7], h->stream));
+  h->src_ms_pending = true;
   h->n_new = n_new;
+  h->last_n_new = n_new;
   if (n_new_out) *n_new_out = n_new;
   return BH_OK;
+}
+
+// Duration of the last run_source, from its events once the second has
+// completed.
+double source_ms(bh_handle* h) {
+  if (h->src_ms_pending) {
+    h->src_ms_pending = false;
+    h->last_source_ms = hipEventSynchronize(h->ev[7]) == hipSuccess
+                            ? elapsed_ms(h->ev[6], h->ev[7]) : 0.0;
+  }
+  return h->last_source_ms;
 }
 
 int bh_source(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_t seed,
@@ -1623,7 +1665,7 @@ int bh_transport(bh_handle* h, double next_dt, double* abs_E, double* track_E,
     stats->n_allocs = take_allocs(h);
     stats->transport_ms = elapsed_ms(h->ev[0], h->ev[1]);
     stats->census_ms = elapsed_ms(h->ev[1], h->ev[2]);
-    stats->source_ms = h->last_source_ms;
+    stats->source_ms = source_ms(h);
     stats->replay_ms = h->replay_timed ? elapsed_ms(h->ev[3], h->ev[1]) : 0.0;
   }
   if (ctrs.stuck)
@@ -1683,7 +1725,7 @@ static int record_step(bh_handle* h, uint64_t cap, bool gen) {
   if (!gen)
     k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
         dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused,
-        &P->fresh, 0.0, 0, h->W.soa, 0, P);
+        &P->fresh, 0.0, 0, h->W.soa, 0, P, nullptr);
   HIPCHK(h, hipGetLastError());
   const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, &P->fresh,
                       unused,  0.0,      cap,       gen ? 1 : 0, 0};
@@ -1834,6 +1876,7 @@ int bh_step_graph(bh_handle* h, uint64_t n_user_photons, double total_E, uint32_
   h->n_cen = tot.n_census;
   h->n_new = 0;
   h->last_source_ms = 0.0;
+  h->src_ms_pending = false;
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
     stats->census_E = tot.census_E;
@@ -2716,7 +2759,7 @@ int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0, bool gen) {
   if (!gen)
     k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
         dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused, inew,
-        0.0, 0, h->W.soa, 0, P);
+        0.0, 0, h->W.soa, 0, P, nullptr);
   HIPCHK(h, hipGetLastError());
   const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, inew,
                       unused,  0.0,      cap,       gen ? 1 : 0, 0};
@@ -2850,6 +2893,7 @@ int resident_step_graph(bh_handle* h, const uint64_t* n_user, uint32_t step, uin
   h->n_cen = tot.n_census;
   h->n_new = 0;
   h->last_source_ms = 0.0;
+  h->src_ms_pending = false;
   if (!K && (pt.ambiguous || h->mat_ranks > 1)) ++h->serial_etot_steps;
   bh_step_stats st;
   std::memset(&st, 0, sizeof st);

@@ -470,18 +470,20 @@ __device__ __forceinline__ uint32_t hinted_cell(const SourceDesc& sd, uint32_t n
 // Without hints (the initial census), two waves find the cells of the
 // block's first and last photon over the whole offset array (~18 dependent
 // loads at 64^3 cells, whose latency the rest of the block waits out).
+// n_ptr: the count is read on the device, bounded by n_new (the room in W).
 __global__ __launch_bounds__(256) void k_source_fill(
     DevCells cells, const double* __restrict__ emission_E, const double* __restrict__ source_E,
     const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ hint, uint64_t n_new, SourceInst one,
     const SourceInst* __restrict__ inst, double dt, int mode, PhotonSoA W, uint64_t out_base,
-    const StepParams* __restrict__ P) {
+    const StepParams* __restrict__ P, const uint64_t* __restrict__ n_ptr) {
   __shared__ uint32_t s_first, s_last;
   if (P) {
     out_base = P->n_cen;
     n_new = P->n_new;
     dt = P->dt;
   }
+  if (n_ptr) n_new = *n_ptr < n_new ? *n_ptr : n_new;
   const uint64_t n_hint = (n_new + kHintStride - 1) / kHintStride;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t p0 = (uint64_t)blockIdx.x * blockDim.x; p0 < n_new; p0 += stride) {

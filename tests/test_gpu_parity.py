@@ -76,14 +76,19 @@ def _cmp_photons(g, o, exact_angle=False):
         assert np.allclose(g["angle"], o["angle"], rtol=0, atol=4e-16)
 
 
+# reserve: the photon store's room when the source runs.  The new photons are
+# written before the host reads their count, bounded by that room: with room
+# for all of them (2e6), too little (1000: a short write, then the store grows
+# and they are written again) or none reserved.
+@pytest.mark.parametrize("reserve", [0, 1000, 2_000_000])
 @pytest.mark.parametrize("deck,n_user", [("cube_40.xml", 200_000),
                                          ("marshak_slab_100.xml", 100_000)])
-def test_source_matches_oracle(deck, n_user):
+def test_source_matches_oracle(deck, n_user, reserve):
     d = _deck(deck)
     sim = oracle.Sim(d)
     cells, ph, clk = _step_inputs(sim)
     census_E = ph["census_E"] if clk["step"] == 1 else None
-    with _engine() as e:
+    with _engine(**({"reserve_photons": reserve} if reserve else {})) as e:
         e.set_mesh_cells(cells)
         n_new = e.source(ph["emission_E"], ph["source_E"], census_E, ph["source_face"],
                          n_user, ph["total_E"], d.seed, clk["step"], 0, clk["dt"])
