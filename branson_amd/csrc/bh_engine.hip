@@ -894,7 +894,7 @@ int ensure_marks(bh_handle* h, uint64_t pc) {
 
 // Buffers and the output record of a census-writing launch over the n
 // photons of W = [census | new]: C has room for all of them and the holes
-// the waves leave, the flags by transport index are cleared, the totals
+// the waves leave, the marks by transport index are cleared, the totals
 // zeroed.  Fill kernels, no memset, so the same code records into a graph
 // (P: sizes read on the device; cap bounds n).
 int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co,
@@ -924,9 +924,9 @@ int census_prepare(bh_handle* h, uint64_t cap, double next_dt, CensusOut& co,
 }
 
 // After the launch: the holes below n_census filled from the photons above
-// it, each census photon's rank in the reference's order (exclusive prefix
-// of the flags over transport index), the census energy (deterministic sum
-// over C) and the totals, into h->d_ctot.
+// it, each census photon's rank in the reference's order (the count of marks
+// below its transport index), the census energy (summed by rank in a fixed
+// split) and the totals, into h->d_ctot.
 int census_finish(bh_handle* h, uint64_t cap, const uint64_t* n_ptr) {
   const CensusSizes z = census_sizes(h, cap);
   TRY(ensure(h, h->hole_len, z.rec_cap));
@@ -1011,7 +1011,7 @@ extern "C" {
 
 namespace {
 // The photon stores for launches of up to `n` photons, and every buffer a
-// step sizes from the stores' capacity (census flags and ranks, scan tiles,
+// step sizes from the stores' capacity (census marks and ranks, scan tiles,
 // hole records, source hints, the FP32 replay list, the event-split lists),
 // allocated together at creation: steps within the reservation then
 // allocate nothing.  Each hipMalloc / hipFree synchronises the device, and

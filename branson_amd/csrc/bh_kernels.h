@@ -27,10 +27,11 @@ struct PhotonSoA {
 };
 
 // Census output of a transport launch: each history that ends as census is
-// written by the kernel itself to a slot of C, its energy stored at its
-// transport index t in flag[t] -- the reference's census order is the order
-// of t, so ranks are a scan of the nonzero flags and the census energy a sum
-// over them in that order -- and exits,
+// written by the kernel itself to a slot of C and marked at its transport
+// index t (mark[t] = 1) -- the reference's census order is the order of t,
+// so a census photon's rank is the count of marks below its t (marks packed
+// 64 to a word, a scan of the word counts, a popcount within the word) and
+// the census energy is summed by rank -- and exits,
 // kills and other endings are counted on the fly.  With `on` false the
 // photons stay in place in W (bh_transport_photons).  W = [census (keys =
 // census ranks) | new photons], transport index t = n_new + key for a census
@@ -1078,10 +1079,6 @@ __global__ void k_census_perm(const uint32_t* __restrict__ key, uint64_t n,
   if (s < n && key[s] < n) perm[key[s]] = (uint32_t)s;
 }
 
-// After a census-writing transport: C.key[s] holds the survivor's transport
-// index t; rank = exclusive prefix of flag over t gives its place in the
-// reference's census order.  count = survivors (on the device); rank holds
-// cap entries.
 // Census marks by transport index, 64 to a word: each word's bit mask and
 // its count (their exclusive scan gives each word's first rank), and the
 // word count into acc->n_words.  The marks are zero up to a whole word past
