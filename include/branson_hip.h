@@ -197,7 +197,9 @@ int bh_set_source_energy(bh_handle *h, const double *emission_E,
  * 207-291), joined ahead of the resident census (join_photon_arrays,
  * src/census_functions.h:20-29).  cycle is the step number; total_E the
  * all-rank photon energy (src/replicated_driver.h:57-59).
- * *n_new (may be NULL) receives the number of new photons. */
+ * *n_new (may be NULL) receives the number of new photons.  The call returns
+ * once that count is known; the photons are written on the handle's stream,
+ * ahead of whatever the handle does next. */
 int bh_source(bh_handle *h, uint64_t n_user_photons, double total_E,
               uint32_t seed, uint32_t cycle, uint32_t rank, double dt,
               uint64_t *n_new);
