@@ -187,6 +187,7 @@ struct bh_handle {
 
   // scratch
   DArr<uint64_t> counts, off, tiles;
+  DArr<SourceCell> src_cell;      // per-cell source photon counts and energies
   DArr<double> tile_E;        // per-tile census energies of the rank scan
   DArr<AosPhoton> aos;
   DArr<double> adj_g, grad_sa, grad_f;  // adjoint: dL/d abs_E in, gradients out
@@ -989,8 +990,8 @@ int materialize_source(bh_handle* h, uint64_t n_new) {
   TRY(grow_store(h, h->W, h->n_cen + n_new, h->n_cen));
   const SourceDesc& d = h->sd;
   k_source_fill<<<grid_for(n_new, 256), 256, 0, h->stream>>>(
-      dev_cells(h), d.em, d.src, d.face, d.off, d.hint, n_new, d.one, d.inst, d.dt, 0,
-      h->W.soa, h->n_cen, nullptr, nullptr);
+      dev_cells(h), d.sc, d.face, d.off, d.hint, n_new, d.one, d.inst, d.dt, 0, h->W.soa,
+      h->n_cen, nullptr, nullptr);
   HIPCHK(h, hipGetLastError());
   h->src_deferred = false;
   return BH_OK;
@@ -1138,6 +1139,7 @@ void bh_destroy(bh_handle* h) {
   release(h->abs_E);
   release(h->track_E);
   release(h->counts);
+  release(h->src_cell);
   release(h->off);
   release(h->tiles);
   release(h->aos);
@@ -1255,6 +1257,7 @@ int bh_set_mesh(bh_handle* h, const bh_cells* c) {
   TRY(ensure(h, h->abs_E, n));
   TRY(ensure(h, h->track_E, n));
   TRY(ensure(h, h->counts, n));  // the source's per-cell counts and offsets
+  TRY(ensure(h, h->src_cell, n));
   TRY(ensure(h, h->soff, n));
   TRY(ensure(h, h->tiles, std::max<uint64_t>(div_up(n, kScanTile), 1)));
   HIPCHK(h, hipMemcpyAsync(h->geo.p, geo.data(), n * sizeof(CellGeo), hipMemcpyHostToDevice,
@@ -1362,12 +1365,13 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
   const uint32_t n = h->n_cells;
   const DevCells cells = dev_cells(h);
   TRY(ensure(h, h->counts, n));
+  TRY(ensure(h, h->src_cell, n));
   TRY(ensure(h, h->soff, n));
   const uint32_t* cinst = h->n_inst ? h->cell_inst.p : nullptr;
   HIPCHK(h, hipEventRecord(h->ev[6], h->stream));
   if (h->has_census_E) {  // step 1: initial census replaces the resident one
     k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->cen.p, nullptr, n, census, icen,
-                                                              cinst, 1, h->counts.p);
+                                                              cinst, 1, h->counts.p, h->src_cell.p);
     if (census_part) {  // initial census energy, block partials
       k_census_partials<<<census_nb, kSumNT, 0, h->stream>>>(h->cen.p, h->counts.p, n,
                                                              census_part);
@@ -1379,14 +1383,14 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
     TRY(grow_store(h, h->W, n_c, 0));
     if (n_c)
       k_source_fill<<<grid_for(n_c, 256), 256, 0, h->stream>>>(
-          cells, h->cen.p, nullptr, h->face.p, h->soff.p, nullptr, n_c, census, icen, dt, 1,
+          cells, h->src_cell.p, h->face.p, h->soff.p, nullptr, n_c, census, icen, dt, 1,
           h->W.soa, 0, nullptr, nullptr);
     HIPCHK(h, hipGetLastError());
     h->n_cen = n_c;
     h->has_census_E = false;
   }
   k_source_counts<<<grid_for(n, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, n, fresh, inew,
-                                                            cinst, 0, h->counts.p);
+                                                            cinst, 0, h->counts.p, h->src_cell.p);
   TRY(scan_u64(h, h->counts.p, n, h->soff.p, 1));
   // The new photons: made inside the persistent transport kernel, or written
   // to W behind the census here.  Written here, the hints and the fill are
@@ -1410,7 +1414,7 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
     k_source_hints<<<std::min<unsigned>(grid_for(div_up(room, kHintStride), 256), 1024), 256, 0,
                      h->stream>>>(h->soff.p, n, room, h->d_total + 1, h->src_hint.p);
     k_source_fill<<<grid_for(std::min(est, room), 256), 256, 0, h->stream>>>(
-        cells, h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, room, fresh, inew, dt, 0,
+        cells, h->src_cell.p, h->face.p, h->soff.p, h->src_hint.p, room, fresh, inew, dt, 0,
         h->W.soa, h->n_cen, nullptr, h->d_total + 1);
     HIPCHK(h, hipGetLastError());
   }
@@ -1419,7 +1423,7 @@ static int run_source(bh_handle* h, const SourceInst& census, const SourceInst& 
   const bool again = n_new > room;  // (or the photons are made in the transport kernel)
   if (again && room) HIPCHK(h, hipStreamSynchronize(h->stream));  // the short fill is done
   if (again) TRY(ensure(h, h->src_hint, div_up(n_new, kHintStride) + 1));
-  h->sd = SourceDesc{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, inew, fresh, dt,
+  h->sd = SourceDesc{h->src_cell.p, h->face.p, h->soff.p, h->src_hint.p, inew, fresh, dt,
                      n_new, 1, 0};
   h->src_deferred = n_new > 0 && (again || !write_now);
   if (again) {
@@ -1717,17 +1721,17 @@ static int record_step(bh_handle* h, uint64_t cap, bool gen) {
                            h->stream));
   const SourceInst unused{};
   k_source_counts<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, nc, unused,
-                                                             &P->fresh, nullptr, 0, h->counts.p);
+                                                             &P->fresh, nullptr, 0, h->counts.p, h->src_cell.p);
   TRY(scan_u64(h, h->counts.p, nc, h->soff.p, 1));
   // the new photons: made inside the transport kernel, or written to W
   k_source_hints<<<std::min<unsigned>(grid_for(div_up(cap, kHintStride), 256), 1024), 256, 0,
                    h->stream>>>(h->soff.p, nc, cap, &P->n_new, h->src_hint.p);
   if (!gen)
     k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
-        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused,
+        dev_cells(h), h->src_cell.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused,
         &P->fresh, 0.0, 0, h->W.soa, 0, P, nullptr);
   HIPCHK(h, hipGetLastError());
-  const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, &P->fresh,
+  const SourceDesc sd{h->src_cell.p, h->face.p, h->soff.p, h->src_hint.p, &P->fresh,
                       unused,  0.0,      cap,       gen ? 1 : 0, 0};
   ZeroList zl;
   TRY(zero_tallies_and_counters(h, &zl));
@@ -2750,7 +2754,7 @@ int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0, bool gen) {
   const uint32_t* cinst = K ? h->cell_inst.p : nullptr;
   SourceInst* inew = rec + (K ? K : 1);
   k_source_counts<<<grid_for(nc, 256), 256, 0, h->stream>>>(h->em.p, h->src.p, nc, unused, inew,
-                                                             cinst, 0, h->counts.p);
+                                                             cinst, 0, h->counts.p, h->src_cell.p);
   TRY(scan_u64(h, h->counts.p, nc, h->soff.p, 1));
   k_step_counts<<<1, 1, 0, h->stream>>>(P, h->d_total + 1);
   // the new photons: made inside the transport kernel, or written to W
@@ -2758,10 +2762,10 @@ int record_resident(bh_handle* h, uint64_t cap, uint64_t n_user0, bool gen) {
                    h->stream>>>(h->soff.p, nc, cap, &P->n_new, h->src_hint.p);
   if (!gen)
     k_source_fill<<<(unsigned)h->n_cu * 16, 256, 0, h->stream>>>(
-        dev_cells(h), h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused, inew,
+        dev_cells(h), h->src_cell.p, h->face.p, h->soff.p, h->src_hint.p, 0, unused, inew,
         0.0, 0, h->W.soa, 0, P, nullptr);
   HIPCHK(h, hipGetLastError());
-  const SourceDesc sd{h->em.p, h->src.p, h->face.p, h->soff.p, h->src_hint.p, inew,
+  const SourceDesc sd{h->src_cell.p, h->face.p, h->soff.p, h->src_hint.p, inew,
                       unused,  0.0,      cap,       gen ? 1 : 0, 0};
   if (K) {  // each instance's census photons before transport
     k_inst_census<<<std::min<unsigned>(grid_for(std::max<uint64_t>(cap, 1), 256), 4096), 256, 0,

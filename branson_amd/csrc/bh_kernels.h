@@ -143,9 +143,18 @@ struct StepParams {
 // the FP64 replay) instead of being written to HBM by a source kernel and
 // read straight back: where photon p of the step's new photons comes from.
 // Photon slots from first_gen on are new photon p = slot - first_gen.
+// Per cell, from the source counts: its emission photons and the energy of
+// each of its emission and face-source photons (initial census: its census
+// photons and their energy), which a photon then reads instead of dividing.
+struct SourceCell {
+  double e_em;    // emission_E / n_em (initial census: census_E / n)
+  double e_src;   // source_E / n_src
+  uint32_t n_em;  // emission (initial census: census) photons
+  uint32_t pad;
+};
+
 struct SourceDesc {
-  const double* em;          // per-cell emission (initial census: census) energies
-  const double* src;         // per-cell face-source energies
+  const SourceCell* sc;      // per-cell photon counts and energies
   const int8_t* face;        // per-cell SOURCE face
   const uint64_t* off;       // exclusive prefix of the per-cell photon counts
   const uint32_t* hint;      // hint[j]: the cell of photon j * kHintStride
@@ -395,13 +404,21 @@ __global__ void k_source_counts(const double* __restrict__ emission_E,
                                 const double* __restrict__ source_E, uint32_t n_cells,
                                 SourceInst one, const SourceInst* __restrict__ inst,
                                 const uint32_t* __restrict__ cell_inst, int mode,
-                                uint64_t* __restrict__ counts) {
+                                uint64_t* __restrict__ counts, SourceCell* __restrict__ sc) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cells) return;
   const SourceInst sp = inst ? inst[cell_inst ? cell_inst[c] : 0u] : one;
-  uint64_t n = photons_for(sp.n_user, emission_E[c], sp.total_E);
-  if (mode == 0) n += photons_for(sp.n_user, source_E[c], sp.total_E);
+  const uint32_t n_em = photons_for(sp.n_user, emission_E[c], sp.total_E);
+  SourceCell o{0.0, 0.0, n_em, 0u};
+  if (n_em) o.e_em = emission_E[c] / (double)n_em;  // a photon's E (src/source.h:169, 265)
+  uint64_t n = n_em;
+  if (mode == 0) {
+    const uint32_t n_src = photons_for(sp.n_user, source_E[c], sp.total_E);
+    if (n_src) o.e_src = source_E[c] / (double)n_src;  // (src/source.h:280)
+    n += n_src;
+  }
   counts[c] = n;
+  sc[c] = o;
 }
 
 // One work-item per new photon p: locate its cell in the exclusive prefix
@@ -413,8 +430,7 @@ __global__ void k_source_counts(const double* __restrict__ emission_E,
 // With `P` set (a whole step captured as a graph), n_new, out_base and dt
 // are read on the device.  Grid-stride loop.
 __device__ __forceinline__ void source_one(const DevCells& cells,
-                                           const double* __restrict__ emission_E,
-                                           const double* __restrict__ source_E,
+                                           const SourceCell* __restrict__ sc,
                                            const int8_t* __restrict__ face_of,
                                            const uint64_t* __restrict__ off, uint64_t p,
                                            uint32_t c, const SourceInst& one,
@@ -473,8 +489,8 @@ __device__ __forceinline__ uint32_t hinted_cell(const SourceDesc& sd, uint32_t n
 // loads at 64^3 cells, whose latency the rest of the block waits out).
 // n_ptr: the count is read on the device, bounded by n_new (the room in W).
 __global__ __launch_bounds__(256) void k_source_fill(
-    DevCells cells, const double* __restrict__ emission_E, const double* __restrict__ source_E,
-    const int8_t* __restrict__ face_of, const uint64_t* __restrict__ off,
+    DevCells cells, const SourceCell* __restrict__ sc, const int8_t* __restrict__ face_of,
+    const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ hint, uint64_t n_new, SourceInst one,
     const SourceInst* __restrict__ inst, double dt, int mode, PhotonSoA W, uint64_t out_base,
     const StepParams* __restrict__ P, const uint64_t* __restrict__ n_ptr) {
@@ -505,8 +521,7 @@ __global__ __launch_bounds__(256) void k_source_fill(
     const uint64_t p = p0 + threadIdx.x;
     if (p < n_new) {
       const uint32_t c = cell_of_photon(off, p, c_first, c_end);
-      source_one(cells, emission_E, source_E, face_of, off, p, c, one, inst, dt, mode, W,
-                 out_base);
+      source_one(cells, sc, face_of, off, p, c, one, inst, dt, mode, W, out_base);
     }
   }
 }
@@ -541,20 +556,17 @@ __device__ __forceinline__ uint64_t source_stream(const SourceInst& sp,
 
 template <class Draw>
 __device__ __forceinline__ SourcePhoton make_source_photon(
-    const DevCells& cells, const double* __restrict__ emission_E,
-    const double* __restrict__ source_E, const int8_t* __restrict__ face_of,
+    const DevCells& cells, const SourceCell* __restrict__ sc, const int8_t* __restrict__ face_of,
     const uint64_t* __restrict__ off, uint64_t p, uint32_t c, const SourceInst& sp, double dt,
     int mode, Draw&& draw) {
   const uint64_t local = p - off[c];
-  const uint64_t n_user = sp.n_user;
-  const double total_E = sp.total_E;
   const CellGeo& g = cells.geo[c];
   const double n0 = g.nodes[0], n1 = g.nodes[1], n2 = g.nodes[2], n3 = g.nodes[3],
                n4 = g.nodes[4], n5 = g.nodes[5];
   SourcePhoton o;
-  const uint32_t n_em = photons_for(n_user, emission_E[c], total_E);
-  if (mode == 1 || local < n_em) {  // set_initial_census_photon / set_emission_photon
-    o.E = emission_E[c] / (double)n_em;
+  const SourceCell k = sc[c];
+  if (mode == 1 || local < k.n_em) {  // set_initial_census_photon / set_emission_photon
+    o.E = k.e_em;
     o.x = n0 + draw() * (n1 - n0);  // get_uniform_position_in_cell
     o.y = n2 + draw() * (n3 - n2);
     o.z = n4 + draw() * (n5 - n4);
@@ -574,8 +586,7 @@ __device__ __forceinline__ SourcePhoton make_source_photon(
       o.ctr = 7;
     }
   } else {  // set_boundary_source_photon
-    const uint32_t n_src = photons_for(n_user, source_E[c], total_E);
-    o.E = source_E[c] / (double)n_src;
+    o.E = k.e_src;
     const int face = face_of[c];
     // get_uniform_position_on_face (src/sampling_functions.h:32-53)
     const double a = draw();
@@ -626,16 +637,15 @@ constexpr int kSourceDraws = 7;  // the most draws a source photon takes
 
 // Draws photon p in cell c takes: 7 for emission, 6 for a face-source or an
 // initial-census photon.
-__device__ __forceinline__ int source_draw_count(const double* __restrict__ emission_E,
+__device__ __forceinline__ int source_draw_count(const SourceCell* __restrict__ sc,
                                                  const uint64_t* __restrict__ off, uint64_t p,
-                                                 uint32_t c, const SourceInst& sp, int mode) {
+                                                 uint32_t c, int mode) {
   if (mode == 1) return 6;
-  return (p - off[c] < photons_for(sp.n_user, emission_E[c], sp.total_E)) ? 7 : 6;
+  return (p - off[c] < sc[c].n_em) ? 7 : 6;
 }
 
 __device__ __forceinline__ void source_one(const DevCells& cells,
-                                           const double* __restrict__ emission_E,
-                                           const double* __restrict__ source_E,
+                                           const SourceCell* __restrict__ sc,
                                            const int8_t* __restrict__ face_of,
                                            const uint64_t* __restrict__ off, uint64_t p,
                                            uint32_t c, const SourceInst& one,
@@ -643,7 +653,7 @@ __device__ __forceinline__ void source_one(const DevCells& cells,
                                            int mode, const PhotonSoA& W, uint64_t out_base) {
   const SourceInst sp = source_record(cells, c, one, inst);
   Rng rng{0ull, source_stream(sp, off, p), sp.ctr_hi};
-  const SourcePhoton o = make_source_photon(cells, emission_E, source_E, face_of, off, p, c, sp,
+  const SourcePhoton o = make_source_photon(cells, sc, face_of, off, p, c, sp,
                                             dt, mode, [&]() { return rng.next(); });
   const uint64_t i = out_base + p;
   // an initial census (mode 1) is in the reference's order by p; new photons

@@ -591,7 +591,7 @@ __device__ __forceinline__ void gen_photon_lane(const DevCells& cells, const Sou
   const uint32_t c = hinted_cell(sd, cells.n_cells, n_new, p);
   const SourceInst sp = source_record(cells, c, sd.one, sd.inst);
   Rng rng{0ull, source_stream(sp, sd.off, p), sp.ctr_hi};
-  const SourcePhoton o = make_source_photon(cells, sd.em, sd.src, sd.face, sd.off, p, c, sp, dt,
+  const SourcePhoton o = make_source_photon(cells, sd.sc, sd.face, sd.off, p, c, sp, dt,
                                             0, [&]() { return rng.next(); });
   live_from_source<R, kLat>(cells, o, sp.ctr_hi, L);
 }
@@ -1214,7 +1214,7 @@ __device__ __forceinline__ bool args_ok(const PhotonSoA& W, const CensusOut& co,
 __device__ __forceinline__ bool args_ok(const PhotonSoA& W, const CensusOut& co,
                                         const SourceDesc& sd, DevCounters* ctrs) {
   const SourceDesc v = arg_source();
-  const bool ok = v.off == sd.off && v.hint == sd.hint && v.em == sd.em && v.on == sd.on &&
+  const bool ok = v.off == sd.off && v.hint == sd.hint && v.sc == sd.sc && v.on == sd.on &&
                   v.n_new == sd.n_new;
   if (!ok && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctrs->bad_args, 1ull);
   return ok && args_ok(W, co, ctrs);
@@ -1487,7 +1487,7 @@ void k_transport_persistent(const PhotonSoA W, const CensusOut co,
           p = idx - first_gen;
           c = hinted_cell(s, cells.n_cells, cr.n_new, p);
           sp = source_record(cells, c, s.one, s.inst);
-          nd = source_draw_count(s.em, s.off, p, c, sp, 0);
+          nd = source_draw_count(s.sc, s.off, p, c, 0);
           stream = source_stream(sp, s.off, p);
           hi = sp.ctr_hi;
         }
@@ -1495,7 +1495,7 @@ void k_transport_persistent(const PhotonSoA W, const CensusOut co,
         wave_gen_draws(gen, nd, stream, hi, gd);
         if (gen) {
           uint32_t j = gd.next;
-          const SourcePhoton o = make_source_photon(cells, s.em, s.src, s.face, s.off, p, c, sp,
+          const SourcePhoton o = make_source_photon(cells, s.sc, s.face, s.off, p, c, sp,
                                                     P ? P->dt : s.dt, 0,
                                                     [&]() { return gd.res[j++]; });
           live_from_source<R, kLat>(cells, o, hi, L);
