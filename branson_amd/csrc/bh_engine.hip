@@ -1698,7 +1698,8 @@ static std::vector<uintptr_t> step_graph_key(const bh_handle* h) {
   auto u = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
   return {u(h->W.slab),      u(h->C.slab),    (uintptr_t)h->W.cap,  (uintptr_t)h->C.cap,
           u(h->geo.p),       u(h->sigma.p),   u(h->next.p),         u(h->em.p),
-          u(h->src.p),       u(h->face.p),    u(h->counts.p),       u(h->off.p),
+          u(h->src.p),       u(h->face.p),    u(h->counts.p),       u(h->src_cell.p),
+          u(h->off.p),
           u(h->soff.p),      u(h->src_hint.p),
           u(h->tiles.p),     u(h->abs_E.p),
           u(h->track_E.p),   u(h->rep_idx.p), u(h->rep_first.p),    u(h->rep_E.p),
