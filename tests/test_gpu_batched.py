@@ -4,8 +4,9 @@ workload of many problems of uneven size.
   * bh_source_instances equals each instance's own make_photons (its seed,
     photon count and energy, streams numbered within the instance), photon
     for photon, concatenated with the cell offsets;
-  * one FP64 transport launch over all instances equals the oracle's
-    transport of the same photons on the concatenated mesh;
+  * one FP64 transport over all instances (one work-item per history,
+    persistent waves, event-split passes) equals the oracle's transport of
+    the same photons on the concatenated mesh;
   * FP32 tracking over the batch: every event as FP64, tallies within the
     FP32 tolerance;
   * a whole batched step with the physics on the device
@@ -88,7 +89,7 @@ def test_batched_source_equals_per_instance_sources():
     assert np.array_equal(seeds, b.seeds[b.instance_of_cell()[got["cell"]]])
 
 
-@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT])
+@pytest.mark.parametrize("algo", [capi.ALGO_HISTORY, capi.ALGO_PERSISTENT, capi.ALGO_EVENT])
 def test_batched_transport_fp64_matches_oracle(algo):
     b = _small_batch()
     em, total_E = _host_energies(b)
