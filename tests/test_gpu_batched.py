@@ -9,6 +9,8 @@ workload of many problems of uneven size.
     the same photons on the concatenated mesh;
   * FP32 tracking over the batch: every event as FP64, tallies within the
     FP32 tolerance;
+  * new photons made inside the transport kernel, each from its instance's
+    record and stream, equal the source kernel's (census bit for bit);
   * a whole batched step with the physics on the device
     (bh_step_resident_instances) on 8 instances spanning the configuration's
     range of sizes (50 to 32768 cells, 1e4 to 1e6 photons) equals the
@@ -136,6 +138,31 @@ def test_batched_instance_lattice_records_equal_general_records():
         assert sl[k] == sg[k], k
     assert np.allclose(al, ag, rtol=1e-12, atol=0)
     assert np.allclose(tl, tg, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_FP32_TRACK])
+def test_batched_photons_made_in_transport_equal_source_kernel(precision):
+    """New photons made by the persistent kernel where it transports them
+    (source_mode SOURCE_IN_TRANSPORT), each from its own instance's record and
+    stream, against the source kernel's: the census bit for bit and in order,
+    the photon counts and census energy identical, tallies to 1e-12."""
+    b = _small_batch()
+    em, total_E = _host_energies(b)
+    n, dt = b.n_cells, b.parts[0].dt
+    out = {}
+    for mode in (capi.SOURCE_KERNEL, capi.SOURCE_IN_TRANSPORT):
+        with _engine_with_batch(b, algorithm=capi.ALGO_PERSISTENT, precision=precision,
+                                source_mode=mode) as e:
+            e.source_instances(em, np.zeros(n), None, np.full(n, -1, dtype=np.int8),
+                               b.n_photons, total_E, 1, 0, dt)
+            a, t, st = e.transport(dt)
+            out[mode] = (a, t, st, e.census())
+    (ak, tk, sk, ck), (ag, tg, sg, cg) = out[capi.SOURCE_KERNEL], out[capi.SOURCE_IN_TRANSPORT]
+    for k in ("n_transported", "n_census", "n_exit", "n_killed", "n_segments", "census_E"):
+        assert sk[k] == sg[k], k
+    assert len(ck) == len(cg) and np.array_equal(ck.view(np.uint8), cg.view(np.uint8))
+    assert np.allclose(ak, ag, rtol=1e-12, atol=0)
+    assert np.allclose(tk, tg, rtol=1e-12, atol=0)
 
 
 def test_batched_fp32_tracking_events_match_fp64():
