@@ -82,3 +82,32 @@ def test_noalloc_rejects_a_call_beyond_its_scratch():
         a_r, t_r, st_r = e.transport_photons(ref)
         a_g, t_g, st_g = e.transport_photons_noalloc(got)
         _same(got, ref, a_g, t_g, a_r, t_r, st_g, st_r)
+
+
+def test_noalloc_equals_transport_photons_on_a_batch():
+    """A batch of twelve problems in one engine, each a lattice (per-instance
+    lattice records, each photon's seed from its instance): the same
+    agreement, twice, the second call smaller."""
+    from branson_amd import problems
+    b = problems.batched_instances(n_instances=12, seed=4, cells_range=(20, 800),
+                                   photons_range=(1000, 20000))
+    em = problems.emission_energy(b.nodes, b.cat("sigma_a"), b.cat("fleck"), b.cat("T"),
+                                  b.parts[0].dt)
+    total_E = np.bincount(b.instance_of_cell(), weights=em, minlength=b.n_instances)
+    n = b.n_cells
+    with capi.Engine(0, algorithm=capi.ALGO_PERSISTENT) as e:
+        e.set_mesh(b.nodes, b.bc, b.next, b.cat("sigma_a"), b.cat("sigma_s"), b.cat("fleck"))
+        e.set_instances(b.first_cell, b.seeds)
+        e.source_instances(em, np.zeros(n), None, np.full(n, -1, dtype=np.int8), b.n_photons,
+                           total_E, 1, 0, b.parts[0].dt)
+        photons = e.photons()
+        n1 = len(photons)
+        n2 = n1 // 3
+        print(f"noalloc batch: n1={n1} n2={n2}", flush=True)
+        for sl in (slice(0, n1), slice(n1 - n2, n1)):
+            ref = np.ascontiguousarray(photons[sl])
+            got = ref.copy()
+            a_r, t_r, st_r = e.transport_photons(ref)
+            a_g, t_g, st_g = e.transport_photons_noalloc(got)
+            assert st_r["lattice"] == 2 and st_g["lattice"] == 2
+            _same(got, ref, a_g, t_g, a_r, t_r, st_g, st_r)
