@@ -3,7 +3,8 @@
   * against the oracle's forward-mode differential-operator tally (the same
     estimator, computed history by history in the other direction): equal to
     round-off for random directions, on a vacuum cube, a reflecting slab lit
-    through a source face, and a 4-group problem;
+    through a source face, a 4-group problem and a batch of twelve problems
+    in one engine;
   * against central finite differences with common random numbers
     (h = 1e-4 relative) on the 32^3 lognormal-opacity workload with 1e6
     photons: within statistical error, batch means over 20 photon batches;
@@ -77,6 +78,37 @@ def test_adjoint_equals_oracle_differential_operator_tally(kind):
         scale = np.abs(g * dabs).sum()
         print(kind, got, ref, scale, flush=True)
         assert abs(got - ref) <= 1e-9 * scale
+    assert st["n_segments"] == segs
+
+
+def test_adjoint_on_a_batch_equals_oracle_differential_operator_tally():
+    """Twelve problems in one engine, each photon drawing from its own
+    instance's seed: the adjoint equals the oracle's differential-operator
+    tally on the concatenated mesh."""
+    b = problems.batched_instances(n_instances=12, seed=4, cells_range=(20, 800),
+                                   photons_range=(1000, 20000))
+    n = b.n_cells
+    em = problems.emission_energy(b.nodes, b.cat("sigma_a"), b.cat("fleck"), b.cat("T"),
+                                  b.parts[0].dt)
+    total_E = np.bincount(b.instance_of_cell(), weights=em, minlength=b.n_instances)
+    cells = oracle.Cells(b.nodes, b.bc, b.next, b.cat("sigma_a").reshape(-1, 1),
+                         b.cat("sigma_s").reshape(-1, 1), b.cat("fleck"))
+    rng = np.random.default_rng(5)
+    g = rng.normal(size=n)
+    with capi.Engine(0) as e:
+        e.set_mesh(b.nodes, b.bc, b.next, b.cat("sigma_a"), b.cat("sigma_s"), b.cat("fleck"))
+        e.set_instances(b.first_cell, b.seeds)
+        e.source_instances(em, np.zeros(n), None, np.full(n, -1, dtype=np.int8), b.n_photons,
+                           total_E, 1, 0, b.parts[0].dt)
+        photons = e.photons()
+        da, df, st = e.adjoint_photons(photons, g)
+    assert st["n_transported"] == len(photons)
+    for _ in range(2):
+        va, vf = rng.normal(size=n), rng.normal(size=n) * 0.1
+        _, dabs, segs = oracle.dos_transport(cells, photons.copy(), va, vf, 8)
+        ref = g @ dabs
+        got = da @ va + df @ vf
+        assert abs(got - ref) <= 1e-9 * np.abs(g * dabs).sum()
     assert st["n_segments"] == segs
 
 
